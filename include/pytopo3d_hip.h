@@ -1,0 +1,173 @@
+/*
+ * pytopo3d_hip.h -- C ABI of the MI355X (gfx950) SIMP hot-path library
+ * (libpytopo3d_hip.so).
+ *
+ * It replaces the per-iteration arithmetic of PyTopo3D's
+ * pytopo3d.core.optimizer.top3d (pytopo3d/core/optimizer.py:578-696).  The
+ * reference has no FFI of its own; every entry point below names the Python
+ * function or call site it stands in for, and INTEGRATION.md shows the ctypes
+ * binding.
+ *
+ * Conventions
+ *   - Array arguments are DEVICE pointers (contiguous, caller-owned), except
+ *     where a parameter is documented as host memory.
+ *   - Element arrays: nele = nelx*nely*nelz doubles, element e = iy + ix*nely
+ *     + iz*nelx*nely (Fortran order of the (nely, nelx, nelz) design array).
+ *   - DOF arrays: ndof = 3*(nelx+1)*(nely+1)*(nelz+1), DOF 3*n + c of node
+ *     n = iy + ix*(nely+1) + iz*(nelx+1)*(nely+1), c = x, y, z.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Work is enqueued
+ *     on it; calls that return a host scalar synchronise it.
+ *   - Return value: 0 ok, 1 invalid argument (the reference raises
+ *     ValueError), 2 HIP/runtime error, 3 linear solve not converged (the
+ *     reference raises RuntimeError, pytopo3d/utils/solver.py:57-58).
+ *     tk_last_error() gives the message of the last failure on this thread.
+ *   - A plan is not thread-safe; distinct plans are independent.  Calls on a
+ *     plan allocate no device memory (all workspaces are made by
+ *     tk_plan_create).
+ */
+#ifndef PYTOPO3D_HIP_H
+#define PYTOPO3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tk_plan tk_plan;
+
+/* Precision of the linear solve / operator. */
+enum {
+  TK_FP64 = 0,       /* FP64 operator, FP64 PCG */
+  TK_MIXED_FP32 = 1  /* FP32 operator and multigrid inside FP64 iterative refinement */
+};
+
+/* tk_solve flags */
+enum {
+  TK_SOLVE_X0 = 1,        /* U holds the initial guess (warm start) */
+  TK_SOLVE_JACOBI = 2,    /* Jacobi instead of the multigrid preconditioner */
+  TK_SOLVE_REUSE = 4      /* reuse the hierarchy built by the last solve (same rho, penal) */
+};
+
+typedef struct tk_options {
+  int smooth_degree;      /* Chebyshev smoothing steps per pre/post smoothing (default 2) */
+  double cheb_lo;         /* smoothing interval, fractions of the lambda_max bound (0.06) */
+  double cheb_hi;         /* (1.0) */
+  int max_levels;         /* multigrid levels (12) */
+  int coarse_max_dof;     /* coarsest grid DOF limit, <= 192 (192) */
+  double inner_rtol;      /* mixed precision: inner FP32 tolerance (1e-3) */
+  int inner_maxit;        /* mixed precision: inner iteration cap per refinement step (200) */
+  int check_every;        /* iterations between host convergence checks (8) */
+} tk_options;
+
+/* Fill *opt with the defaults. */
+void tk_options_default(tk_options* opt);
+
+/* Create a plan for a nelx x nely x nelz grid.
+ * KE: HOST pointer, 576 doubles, row-major 24x24 element stiffness in the
+ *     corner order of pytopo3d/utils/assembly.py:17-26 (lk_H8,
+ *     pytopo3d/utils/stiffness.py:12-130).
+ * rmin: filter radius; builds the stencil form of H and Hs
+ *     (replaces build_filter, pytopo3d/utils/filter.py:122-228).
+ * opt: NULL for defaults. */
+int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,
+                   const tk_options* opt);
+int tk_plan_destroy(tk_plan* plan);
+
+/* Masks (device pointers, copied into the plan):
+ * elem_flags: nele bytes, bit0 = protected solid, bit1 = protected void
+ *     (the masks of pytopo3d/core/optimizer.py:343-405, void wins);
+ * node_fixed: nnode bytes, bit c = DOF c of the node is fixed
+ *     (build_supports, pytopo3d/utils/assembly.py:224-348).
+ * Returns the free-element count in *nfree (host). */
+int tk_plan_set_masks(tk_plan* plan, const uint8_t* elem_flags, const uint8_t* node_fixed,
+                      long long* nfree, void* stream);
+
+/* Plan geometry: writes nlev and up to cap*3 element dims (nelx,nely,nelz per level). */
+int tk_plan_levels(const tk_plan* plan, int* nlev, int* dims, int cap);
+/* Stencil table (HOST outputs): dy,dx,dz per entry and the weight; returns count in *n. */
+int tk_plan_stencil(const tk_plan* plan, int* offsets, double* weights, int cap, int* n);
+/* Copy the filter row sums Hs (device output, nele doubles). */
+int tk_filter_row_sums(tk_plan* plan, double* Hs, void* stream);
+
+/* rho_f = (H x)/Hs, rho = Heaviside projection clamped to [0,1] with protected
+ * solid/void forced to 1/0, drho = d rho / d rho_f (0 on protected elements).
+ * Replaces build_physical_density + heaviside_projection
+ * (pytopo3d/utils/filter.py:26-100).  Any output may be NULL. */
+int tk_filter_project(tk_plan* plan, const double* x, double beta, double eta, double* rho_f,
+                      double* rho, double* drho, void* stream);
+
+/* dc_dx = H^T((g_c * drho)/Hs), dv_dx = H^T((g_v * drho)/Hs).
+ * g_v == NULL uses the free-element indicator (dv/drho of optimizer.py:613).
+ * dv_dx may be NULL (then g_v is ignored).
+ * Replaces apply_density_filter_chain_rule (pytopo3d/utils/filter.py:103-119)
+ * at pytopo3d/core/optimizer.py:614-625. */
+int tk_filter_transpose2(tk_plan* plan, const double* g_c, const double* g_v,
+                         const double* drho, double* dc_dx, double* dv_dx, void* stream);
+
+/* y = K(rho) u on the free DOFs (identity rows on fixed DOFs), K = sum_e
+ * (1e-9 + (1-1e-9) rho_e^penal) KE.  prec TK_FP64: u, y double*; TK_MIXED_FP32:
+ * float*.  Replaces the assembly + Kff of pytopo3d/core/optimizer.py:584-591. */
+int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const void* u, void* y,
+               void* stream);
+/* diag(K(rho)) (1 on fixed DOFs), ndof doubles. */
+int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void* stream);
+
+/* Solve K(rho) U = F on the free DOFs, U = 0 on fixed DOFs.
+ * Replaces get_solver + the solve at pytopo3d/core/optimizer.py:591-594
+ * (pytopo3d/utils/solver.py:18-73).  rtol: relative residual target; iters_out
+ * and relres_out (HOST, may be NULL) receive the iteration count and the final
+ * relative residual.  Returns 3 when rtol is not reached. */
+int tk_solve(tk_plan* plan, int prec, const double* rho, double penal, const double* F,
+             double* U, double rtol, int maxit, int flags, int* iters_out, double* relres_out,
+             void* stream);
+
+/* Element energies and sensitivities from a displacement field:
+ * ce = u_e.KE.u_e (element_compliance, pytopo3d/core/compliance.py:18-75),
+ * c = sum_e s_e ce (optimizer.py:601-605), dc_drho = -p(1-1e-9)rho^(p-1) ce
+ * (optimizer.py:607-612).  If dc_dx/dv_dx are given (drho required) the chain
+ * rule of optimizer.py:614-625 is applied too.  Outputs may be NULL.
+ * c_out (HOST, may be NULL): if given the stream is synchronised. */
+int tk_energy(tk_plan* plan, const double* U, const double* rho, const double* drho, double penal,
+              double* ce, double* dc_drho, double* dc_dx, double* dv_dx, double* c_out,
+              void* stream);
+
+/* Projected-volume OC update (optimality_criteria_update_projected,
+ * pytopo3d/utils/oc_update.py:25-124), the bisection entirely on the device.
+ * x, dc_dx, dv_dx: nele doubles; x_new: output.  target: free-volume target
+ * (a sum over free elements).  steps_out (HOST, may be NULL; synchronises). */
+int tk_oc_update(tk_plan* plan, const double* x, const double* dc_dx, const double* dv_dx,
+                 double move, double target, double beta, double eta, int max_steps, double tol,
+                 double* x_new, int* steps_out, void* stream);
+
+/* Statistics over free elements (pytopo3d/core/optimizer.py:642-655):
+ * change = max|x_new - x_old| (x_old/x_new may be NULL: change = -inf),
+ * vol_sum = sum rho, gray = #(0.05 < rho < 0.95).  Left on the device; read
+ * with tk_read_results. */
+int tk_design_stats(tk_plan* plan, const double* x_old, const double* x_new, const double* rho,
+                    void* stream);
+
+/* Mean free physical density of the all-void and all-solid free designs at
+ * beta (achievable_free_volume_bounds, pytopo3d/core/optimizer.py:501-531).
+ * HOST outputs; synchronises. */
+int tk_volume_bounds(tk_plan* plan, double beta, double eta, double* vmin, double* vmax,
+                     void* stream);
+
+/* Synchronise and copy the device results (HOST out[0..n)):
+ * [0] c, [1] change, [2] vol_sum, [3] gray count, [4] OC steps, [5] OC lambda,
+ * [6] OC volume, [7] OC ratio scale, [8] nfree. */
+int tk_read_results(tk_plan* plan, double* out, int n, void* stream);
+
+/* Work counters (HOST out[0..n)): [0] FP64 fine operator applications,
+ * [1] FP32 fine applications, [2] Krylov iterations, [3] refinement steps,
+ * [4] solves, [5] OC bisection steps, [6] filter passes. */
+int tk_plan_counters(const tk_plan* plan, long long* out, int n);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* tk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYTOPO3D_HIP_H */

@@ -1,0 +1,257 @@
+"""ctypes binding of libpytopo3d_hip.so (the C ABI in include/pytopo3d_hip.h).
+
+Device buffers are torch tensors on the current HIP device; their data
+pointers and torch's current stream are handed to the library.  There is no
+fallback: if the library or a GPU is missing, :func:`lib` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpytopo3d_hip.so")
+
+TK_FP64 = 0
+TK_MIXED_FP32 = 1
+TK_SOLVE_X0 = 1
+TK_SOLVE_JACOBI = 2
+TK_SOLVE_REUSE = 4
+
+PRECISIONS = {"fp64": TK_FP64, "mixed": TK_MIXED_FP32, "fp32": TK_MIXED_FP32,
+              "mixed_fp32": TK_MIXED_FP32}
+
+
+class SolveNotConverged(RuntimeError):
+    """The matrix-free Krylov solve missed its tolerance (the reference raises
+    RuntimeError("CG failed to converge ..."), pytopo3d/utils/solver.py:57-58)."""
+
+
+class TkOptions(ctypes.Structure):
+    _fields_ = [
+        ("smooth_degree", ctypes.c_int),
+        ("cheb_lo", ctypes.c_double),
+        ("cheb_hi", ctypes.c_double),
+        ("max_levels", ctypes.c_int),
+        ("coarse_max_dof", ctypes.c_int),
+        ("inner_rtol", ctypes.c_double),
+        ("inner_maxit", ctypes.c_int),
+        ("check_every", ctypes.c_int),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SIGS = {
+    "tk_options_default": (None, [ctypes.POINTER(TkOptions)]),
+    "tk_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, ctypes.POINTER(_D), _D,
+                            ctypes.POINTER(TkOptions)]),
+    "tk_plan_destroy": (_I, [_P]),
+    "tk_plan_set_masks": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_longlong), _P]),
+    "tk_plan_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
+    "tk_plan_stencil": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D), _I, ctypes.POINTER(_I)]),
+    "tk_filter_row_sums": (_I, [_P, _P, _P]),
+    "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
+    "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "tk_apply_K": (_I, [_P, _I, _P, _D, _P, _P, _P]),
+    "tk_diag_K": (_I, [_P, _P, _D, _P, _P]),
+    "tk_solve": (_I, [_P, _I, _P, _D, _P, _P, _D, _I, _I, ctypes.POINTER(_I),
+                      ctypes.POINTER(_D), _P]),
+    "tk_energy": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+    "tk_oc_update": (_I, [_P, _P, _P, _P, _D, _D, _D, _D, _I, _D, _P, ctypes.POINTER(_I), _P]),
+    "tk_design_stats": (_I, [_P, _P, _P, _P, _P]),
+    "tk_volume_bounds": (_I, [_P, _D, _D, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
+    "tk_read_results": (_I, [_P, ctypes.POINTER(_D), _I, _P]),
+    "tk_plan_counters": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I]),
+    "tk_last_error": (ctypes.c_char_p, []),
+}
+EXPORTED = tuple(_SIGS)
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the shared library and declare every entry point (no GPU needed)."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m pytopo3d_amd.build` "
+            "(there is no CPU fallback)")
+    cdll = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(cdll, name)
+        fn.restype = res
+        fn.argtypes = args
+    return cdll
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("pytopo3d_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        _lib = load_library()
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = (lib().tk_last_error() or b"").decode(errors="replace")
+    if rc == 1:
+        raise ValueError(msg)
+    if rc == 3:
+        raise SolveNotConverged(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("device tensors must be contiguous CUDA/HIP tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Plan:
+    """One problem size: stencil filter, masks, multigrid hierarchy, workspaces."""
+
+    def __init__(self, nelx: int, nely: int, nelz: int, KE: np.ndarray, rmin: float,
+                 options: Optional[dict] = None):
+        L = lib()
+        ke = np.ascontiguousarray(np.asarray(KE, dtype=np.float64).reshape(24, 24))
+        opt = TkOptions()
+        L.tk_options_default(ctypes.byref(opt))
+        for k, v in (options or {}).items():
+            if not hasattr(opt, k):
+                raise ValueError(f"unknown solver option {k!r}")
+            setattr(opt, k, v)
+        h = _P()
+        _check(L.tk_plan_create(ctypes.byref(h), int(nelx), int(nely), int(nelz),
+                                ke.ctypes.data_as(ctypes.POINTER(_D)), float(rmin),
+                                ctypes.byref(opt)))
+        self._h = h
+        self.nelx, self.nely, self.nelz = int(nelx), int(nely), int(nelz)
+        self.nele = self.nelx * self.nely * self.nelz
+        self.nnode = (self.nelx + 1) * (self.nely + 1) * (self.nelz + 1)
+        self.ndof = 3 * self.nnode
+        self.nfree = self.nele
+        self.options = {k: getattr(opt, k) for k, _ in TkOptions._fields_}
+
+    # -- lifecycle -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().tk_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up ----------------------------------------------------------
+    def set_masks(self, elem_flags: torch.Tensor, node_fixed: torch.Tensor) -> int:
+        assert elem_flags.dtype == torch.uint8 and elem_flags.numel() == self.nele
+        assert node_fixed.dtype == torch.uint8 and node_fixed.numel() == self.nnode
+        n = ctypes.c_longlong()
+        _check(lib().tk_plan_set_masks(self._h, _ptr(elem_flags), _ptr(node_fixed),
+                                       ctypes.byref(n), _stream()))
+        self.nfree = int(n.value)
+        return self.nfree
+
+    def levels(self):
+        n = _I()
+        dims = (_I * 36)()
+        _check(lib().tk_plan_levels(self._h, ctypes.byref(n), dims, 12))
+        return [tuple(dims[3 * i:3 * i + 3]) for i in range(n.value)]
+
+    def stencil(self):
+        n = _I()
+        _check(lib().tk_plan_stencil(self._h, None, None, 0, ctypes.byref(n)))
+        off = np.zeros((n.value, 3), dtype=np.int32)
+        w = np.zeros(n.value)
+        _check(lib().tk_plan_stencil(self._h, off.ctypes.data_as(ctypes.POINTER(_I)),
+                                     w.ctypes.data_as(ctypes.POINTER(_D)), n.value,
+                                     ctypes.byref(n)))
+        return off, w   # columns (dy, dx, dz)
+
+    def row_sums(self, out: torch.Tensor) -> torch.Tensor:
+        _check(lib().tk_filter_row_sums(self._h, _ptr(out), _stream()))
+        return out
+
+    # -- hot path --------------------------------------------------------
+    def filter_project(self, x, beta, eta, rho_f=None, rho=None, drho=None):
+        _check(lib().tk_filter_project(self._h, _ptr(x), float(beta), float(eta), _ptr(rho_f),
+                                       _ptr(rho), _ptr(drho), _stream()))
+
+    def filter_transpose2(self, g_c, g_v, drho, dc_dx, dv_dx=None):
+        _check(lib().tk_filter_transpose2(self._h, _ptr(g_c), _ptr(g_v), _ptr(drho),
+                                          _ptr(dc_dx), _ptr(dv_dx), _stream()))
+
+    def apply_K(self, rho, penal, u, y, precision: str = "fp64"):
+        _check(lib().tk_apply_K(self._h, PRECISIONS[precision], _ptr(rho), float(penal),
+                                _ptr(u), _ptr(y), _stream()))
+
+    def diag_K(self, rho, penal, diag):
+        _check(lib().tk_diag_K(self._h, _ptr(rho), float(penal), _ptr(diag), _stream()))
+
+    def solve(self, rho, penal, F, U, rtol=1e-10, maxit=5000, precision="fp64",
+              warm_start=False, jacobi=False, reuse=False):
+        flags = (TK_SOLVE_X0 if warm_start else 0) | (TK_SOLVE_JACOBI if jacobi else 0) | (
+            TK_SOLVE_REUSE if reuse else 0)
+        it = _I()
+        rr = _D()
+        _check(lib().tk_solve(self._h, PRECISIONS[precision], _ptr(rho), float(penal), _ptr(F),
+                              _ptr(U), float(rtol), int(maxit), flags, ctypes.byref(it),
+                              ctypes.byref(rr), _stream()))
+        return int(it.value), float(rr.value)
+
+    def energy(self, U, rho, penal, drho=None, ce=None, dc_drho=None, dc_dx=None, dv_dx=None,
+               sync=True):
+        c = _D()
+        _check(lib().tk_energy(self._h, _ptr(U), _ptr(rho), _ptr(drho), float(penal), _ptr(ce),
+                               _ptr(dc_drho), _ptr(dc_dx), _ptr(dv_dx),
+                               ctypes.byref(c) if sync else None, _stream()))
+        return float(c.value) if sync else None
+
+    def oc_update(self, x, dc_dx, dv_dx, move, target, beta, eta, x_new, max_steps=80,
+                  tol=1e-8, sync=False):
+        st = _I()
+        _check(lib().tk_oc_update(self._h, _ptr(x), _ptr(dc_dx), _ptr(dv_dx), float(move),
+                                  float(target), float(beta), float(eta), int(max_steps),
+                                  float(tol), _ptr(x_new), ctypes.byref(st) if sync else None,
+                                  _stream()))
+        return int(st.value) if sync else None
+
+    def design_stats(self, rho, x_old=None, x_new=None):
+        _check(lib().tk_design_stats(self._h, _ptr(x_old), _ptr(x_new), _ptr(rho), _stream()))
+
+    def volume_bounds(self, beta, eta):
+        lo, hi = _D(), _D()
+        _check(lib().tk_volume_bounds(self._h, float(beta), float(eta), ctypes.byref(lo),
+                                      ctypes.byref(hi), _stream()))
+        return float(lo.value), float(hi.value)
+
+    def results(self) -> dict:
+        out = (_D * 9)()
+        _check(lib().tk_read_results(self._h, out, 9, _stream()))
+        keys = ("c", "change", "vol_sum", "gray_cnt", "oc_steps", "oc_lambda", "oc_volume",
+                "ratio_scale", "nfree")
+        return dict(zip(keys, list(out)))
+
+    def counters(self) -> dict:
+        out = (ctypes.c_longlong * 7)()
+        _check(lib().tk_plan_counters(self._h, out, 7))
+        keys = ("applies64", "applies32", "cg_iters", "ir_steps", "solves", "oc_steps",
+                "filter_passes")
+        return dict(zip(keys, list(out)))

@@ -1,0 +1,174 @@
+// Shared definitions of the MI355X (gfx950) SIMP hot-path library.
+//
+// Layout conventions (identical to PyTopo3D's, pytopo3d/utils/filter.py:169-174
+// and pytopo3d/utils/assembly.py:121-129):
+//   element e = iy + ix*nely + iz*nelx*nely        (Fortran order of (nely,nelx,nelz))
+//   node    n = iy + ix*(nely+1) + iz*(nelx+1)*(nely+1)
+//   DOF       = 3*n + c, c in {x, y, z}            (array-of-structs, 24 B per node)
+// H8 local corner a has (x, y, z) offset kCornerX/Y/Z[a]
+// (pytopo3d/utils/assembly.py:17-26).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace tk {
+
+constexpr int kBlock = 256;           // default workgroup size (4 waves of 64)
+constexpr int kMaxLevels = 12;        // multigrid levels
+constexpr int kCoarseMaxDof = 192;    // coarsest level is factorised densely in LDS
+constexpr int kMaxStencil = 1024;     // density-filter stencil entries (rmin <= ~6)
+
+// element flags
+constexpr uint8_t kSolid = 1;         // protected solid: rho = 1
+constexpr uint8_t kVoid = 2;          // protected void (obstacle): rho = 0, wins over solid
+
+__host__ __device__ constexpr int corner_x(int a) { return (0xCC >> a) & 1; }  // 0,0,1,1,0,0,1,1
+__host__ __device__ constexpr int corner_y(int a) { return (0x66 >> a) & 1; }  // 0,1,1,0,0,1,1,0
+__host__ __device__ constexpr int corner_z(int a) { return (0xF0 >> a) & 1; }  // 0,0,0,0,1,1,1,1
+
+// Geometry of one grid (fine level or a multigrid level).
+struct Grid {
+  int nelx, nely, nelz;   // elements
+  int nnx, nny, nnz;      // nodes = elements + 1
+  int nele, nnode, ndof;
+};
+
+inline Grid make_grid(int nelx, int nely, int nelz) {
+  Grid g;
+  g.nelx = nelx; g.nely = nely; g.nelz = nelz;
+  g.nnx = nelx + 1; g.nny = nely + 1; g.nnz = nelz + 1;
+  g.nele = nelx * nely * nelz;
+  g.nnode = g.nnx * g.nny * g.nnz;
+  g.ndof = 3 * g.nnode;
+  return g;
+}
+
+// Device-resident scalars of one plan.  Kernels read and write these, so the
+// host only synchronises when a caller asks for a scalar result.
+struct Scalars {
+  // conjugate gradients
+  double rz, pq, alpha, beta, rr, bb, tol2, rr0;
+  int iter, done, maxit, stalled;
+  // optimality-criteria bisection
+  double lo, hi, lam, best_lam, best_err, target, V, ratio_scale, tol;
+  int steps, oc_done, max_steps, pad1;
+  // general results
+  double c;            // compliance
+  double change;       // max |x_new - x| over free elements
+  double vol_sum;      // sum of rho over free elements
+  double gray_cnt;     // count of 0.05 < rho < 0.95 over free elements
+  double sum0, sum1;   // scratch sums
+  double norm0, norm1; // scratch norms
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions.  Partial results are combined in a fixed order
+// (wave shuffle tree, then waves, then blocks strided by thread), so equal
+// inputs and launch shapes give bitwise equal results on every run.
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] += __shfl_down(v[i], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[i][wid] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = sh[i][0];
+      for (int w = 1; w < nw; ++w) s += sh[i][w];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double sh[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = sh[0];
+    for (int w = 1; w < nw; ++w) s = fmax(s, sh[w]);
+    v = s;
+  }
+  __syncthreads();
+  return v;
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide sum of NV values.  Every thread of every block must call it (no
+// early returns before it).  The last block to finish calls fin(total) from
+// thread 0 and re-arms the counter.  partials holds NV * gridDim.x doubles.
+template <int NV, class Fin>
+__device__ __forceinline__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter,
+                                         Fin fin) {
+  block_sum<NV>(v);
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) partials[i * gridDim.x + blockIdx.x] = v[i];
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (am_last) {
+    __threadfence();
+    double s[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) s[i] += load_coherent(partials + i * gridDim.x + b);
+    block_sum<NV>(s);
+    if (threadIdx.x == 0) {
+      *counter = 0u;
+      fin(s);
+    }
+  }
+}
+
+template <class Fin>
+__device__ __forceinline__ void grid_max(double v, double* partials, unsigned* counter, Fin fin) {
+  v = block_max(v);
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v;
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (am_last) {
+    __threadfence();
+    double s = -INFINITY;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+      s = fmax(s, load_coherent(partials + b));
+    s = block_max(s);
+    if (threadIdx.x == 0) {
+      *counter = 0u;
+      fin(s);
+    }
+  }
+}
+
+inline int nblocks(long long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace tk

@@ -1,0 +1,96 @@
+// Element energies, compliance and the compliance / volume sensitivities.
+//   ce[e] = u_e . KE . u_e                        pytopo3d/core/compliance.py:18-75
+//   c     = sum_e (Emin + (1-Emin) rho^p) ce[e]   pytopo3d/core/optimizer.py:596-605
+//   dc/drho = -p (1-Emin) rho^(p-1) ce            pytopo3d/core/optimizer.py:607-612
+//   chain-rule weights (g * drho)/Hs for g = dc/drho and g = free mask
+//                                                 pytopo3d/utils/filter.py:115-118
+// The transposed filter that finishes the chain rule is k_filter_transpose2.
+#include "tk_plan.hpp"
+
+namespace tk {
+
+constexpr double kEminE = 1.0e-9;
+
+__global__ __launch_bounds__(kBlock) void k_energy(
+    Grid g, const double* __restrict__ KE, const double* __restrict__ U,
+    const double* __restrict__ rho, const double* __restrict__ drho,
+    const double* __restrict__ Hs, const uint8_t* __restrict__ fl, double penal, double c0,
+    double* __restrict__ ce_out, double* __restrict__ dc_out, double* __restrict__ wc,
+    double* __restrict__ wv, double* partials, unsigned* counter, Scalars* sc) {
+  __shared__ double ke[576];
+  for (int i = threadIdx.x; i < 576; i += blockDim.x) ke[i] = KE[i];
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double part[1] = {0.0};
+  if (e < g.nele) {
+    const int iy = e % g.nely, t = e / g.nely, ix = t % g.nelx, iz = t / g.nelx;
+    double u[24];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int m = (iy + corner_y(a)) + (ix + corner_x(a)) * g.nny +
+                    (iz + corner_z(a)) * g.nnx * g.nny;
+      u[3 * a] = U[3 * m];
+      u[3 * a + 1] = U[3 * m + 1];
+      u[3 * a + 2] = U[3 * m + 2];
+    }
+    double ce = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+      double row = 0.0;
+#pragma unroll
+      for (int j = 0; j < 24; ++j) row += ke[i * 24 + j] * u[j];
+      ce += u[i] * row;
+    }
+    const double r = rho[e];
+    const double s = kEminE + (1.0 - kEminE) * pow(r, penal);
+    part[0] = s * ce;
+    const double dc = c0 * pow(r, penal - 1.0) * ce;
+    if (ce_out) ce_out[e] = ce;
+    if (dc_out) dc_out[e] = dc;
+    if (wc) {
+      const double d = drho[e], h = Hs[e];
+      wc[e] = (dc * d) / h;
+      wv[e] = ((fl[e] == 0 ? 1.0 : 0.0) * d) / h;
+    }
+  }
+  grid_sum<1>(part, partials, counter, [&](double (&t)[1]) { sc->c = t[0]; });
+}
+
+void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
+                   double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s) {
+  const double c0 = -penal * (1.0 - kEminE);
+  hipLaunchKernelGGL(k_energy, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.KE64, U, rho,
+                     drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials, P.counter,
+                     P.sc);
+}
+
+}  // namespace tk
+
+namespace tk {
+
+// wc = (g_c * drho)/Hs, wv = (g_v * drho)/Hs with g_v = free indicator when null
+// (pytopo3d/utils/filter.py:115-118).
+__global__ __launch_bounds__(kBlock) void k_chain_weights(int n, const double* __restrict__ gc,
+                                                           const double* __restrict__ gv,
+                                                           const double* __restrict__ drho,
+                                                           const double* __restrict__ Hs,
+                                                           const uint8_t* __restrict__ fl,
+                                                           double* __restrict__ wc,
+                                                           double* __restrict__ wv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double d = drho[e], h = Hs[e];
+  wc[e] = (gc[e] * d) / h;
+  if (wv) {
+    const double g = gv ? gv[e] : (fl[e] == 0 ? 1.0 : 0.0);
+    wv[e] = (g * d) / h;
+  }
+}
+
+void launch_chain_weights(Plan& P, const double* gc, const double* gv, const double* drho,
+                          double* wc, double* wv, hipStream_t s) {
+  hipLaunchKernelGGL(k_chain_weights, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele, gc,
+                     gv, drho, P.Hs, P.eflags, wc, wv);
+}
+
+}  // namespace tk

@@ -1,0 +1,179 @@
+// Plan object: everything one problem size needs on the device, allocated
+// once at creation (stencil table, Hs, masks, multigrid hierarchy, Krylov
+// vectors, reduction scratch).  Calls on a plan allocate nothing.
+#pragma once
+
+#include "tk_common.hpp"
+
+#include <string>
+#include <vector>
+
+namespace tk {
+
+template <typename T>
+struct LevelBuf {
+  T* s = nullptr;      // element stiffness scale, s = Emin + (1 - Emin) rho^p (fine)
+  T* dinv = nullptr;   // inverse diagonal (1 on fixed DOFs)
+  T* x = nullptr;      // iterate
+  T* xa = nullptr;     // ping-pong partner of x
+  T* b = nullptr;      // right-hand side (levels >= 1)
+  T* r = nullptr;      // residual
+  T* d = nullptr;      // Chebyshev direction
+};
+
+struct Level {
+  Grid g{};
+  uint8_t* fixed = nullptr;      // per node, bit c set = DOF c fixed
+  LevelBuf<double> f64;
+  LevelBuf<float> f32;
+};
+
+struct Options {
+  int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing
+  double cheb_lo = 0.06;         // smoothing interval [cheb_lo, cheb_hi] * lambda_max bound
+  double cheb_hi = 1.0;
+  int max_levels = kMaxLevels;
+  int coarse_max_dof = kCoarseMaxDof;
+  double inner_rtol = 1e-3;      // mixed precision: inner FP32 solve tolerance
+  int inner_maxit = 200;
+  int check_every = 8;           // Krylov convergence is read back every N iterations
+};
+
+struct Counters {
+  long long applies64 = 0, applies32 = 0;      // fine-level operator applications
+  long long cg_iters = 0, ir_steps = 0, solves = 0;
+  long long oc_steps = 0, filter_passes = 0;
+};
+
+struct Plan {
+  Grid g{};
+  double KE[576];                 // host copy, row-major
+  double* KE64 = nullptr;         // device
+  float* KE32 = nullptr;
+  double lam_bound = 0.0;         // lambda_max(diag(KE)^-1 KE): bound of lambda_max(D^-1 K)
+  // density filter
+  double rmin = 0.0;
+  int nsten = 0, R = 0;
+  int* sten_off = nullptr;        // packed (dy+16) | (dx+16)<<8 | (dz+16)<<16
+  double* sten_w = nullptr;
+  double* Hs = nullptr;
+  uint8_t* eflags = nullptr;      // per element, kSolid | kVoid
+  long long nfree = 0;
+  bool masks_set = false;
+  // multigrid hierarchy
+  int nlev = 0;
+  Level lv[kMaxLevels];
+  int ncoarse = 0;                // coarsest DOF count
+  double* coarse_inv64 = nullptr; // dense inverse of the coarsest operator (ncoarse^2)
+  float* coarse_inv32 = nullptr;
+  double* coarse_L = nullptr;     // packed Cholesky factor scratch
+  // Krylov vectors (fine level)
+  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *x64 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  // element work arrays
+  double* ework0 = nullptr;       // OC ratio / design fills
+  double* ework1 = nullptr;       // OC candidate
+  double* ework2 = nullptr;       // chain-rule weights (compliance)
+  double* ework3 = nullptr;       // chain-rule weights (volume)
+  // reductions
+  double* partials = nullptr;
+  int partials_len = 0;
+  unsigned* counter = nullptr;
+  Scalars* sc = nullptr;          // device scalars
+  Scalars* sc_host = nullptr;     // pinned mirror
+  Options opt;
+  Counters cnt;
+  // bookkeeping of the last solve
+  double last_penal = -1.0;
+  int lds_limit = 0;              // per-workgroup LDS bytes of the device
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------
+// filter
+void launch_filter_hs(Plan& P, hipStream_t s);
+void launch_filter_project(Plan& P, const double* x, double beta, double eta, double tb_eta,
+                           double den, double* rf, double* rho, double* drho, hipStream_t s);
+void launch_filter_transpose2(Plan& P, const double* wc, const double* wv, double* oc,
+                              double* ov, hipStream_t s);
+void launch_free_sum_projected(Plan& P, const double* x, double beta, double eta, double tb_eta,
+                               double den, int slot, hipStream_t s);
+void launch_bounds_design(Plan& P, int upper, double* out, hipStream_t s);
+void launch_count_free(Plan& P, hipStream_t s);
+// operator / multigrid
+template <typename T>
+void launch_stiffness_scale(Plan& P, const double* rho, double penal, T* s_out, hipStream_t s);
+template <typename T>
+void launch_coarsen_scale(const Level& f, const Level& c, const T* sf, T* sc, hipStream_t s);
+void launch_coarsen_fixed(const Level& f, const Level& c, hipStream_t s);
+template <typename T>
+void launch_diag_inv(Plan& P, const Level& L, const T* sl, T* dinv, T* diag_out, hipStream_t s);
+template <typename T>
+void launch_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, double* dot_out_slot,
+                  int with_dot, hipStream_t s);
+template <typename T>
+void launch_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T* r,
+                     hipStream_t s, const int* done);
+template <typename T>
+void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
+                       const T* x, T* xo, T* d, double c0, int x_zero, hipStream_t s);
+template <typename T>
+void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
+                      const T* x, T* xo, T* d, double c1, double c2, hipStream_t s);
+template <typename T>
+void launch_restrict(const Level& f, const Level& c, const T* rf, T* bc, hipStream_t s,
+                     const int* done);
+template <typename T>
+void launch_prolong_add(const Level& c, const Level& f, const T* xc, T* xf, hipStream_t s,
+                        const int* done);
+hipError_t coarse_setup_attr(size_t lds);
+void launch_coarse_setup(Plan& P, hipStream_t s);
+template <typename T>
+void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);
+// Krylov vector kernels
+template <typename T>
+void launch_cg_init(Plan& P, const T* b, T* r, T* x, int x_zero, hipStream_t s);
+template <typename T>
+void launch_cg_update(Plan& P, T* x, T* r, const T* p, const T* q, hipStream_t s);
+template <typename T>
+void launch_cg_rz(Plan& P, const T* r, const T* z, int first, hipStream_t s);
+template <typename T>
+void launch_cg_dir(Plan& P, const T* z, T* p, int first, hipStream_t s);
+template <typename T>
+void launch_jacobi(Plan& P, const T* dinv, const T* r, T* z, hipStream_t s);
+void launch_ir_residual(Plan& P, const double* F, const double* x, double* r64, float* r32,
+                        hipStream_t s);
+void launch_ir_scale_r(Plan& P, const double* r64, float* r32, hipStream_t s);
+void launch_ir_correct(Plan& P, double* x, const float* d, hipStream_t s);
+void launch_norm2(Plan& P, const double* v, int n, int slot, hipStream_t s);
+// energy
+void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
+                   double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s);
+// optimality criteria
+void launch_oc_ratio(Plan& P, const double* dc, const double* dv, double* ratio, hipStream_t s);
+void launch_oc_normalize(Plan& P, double* ratio, hipStream_t s);
+void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double move, int use_best,
+                         double* cand, hipStream_t s);
+void launch_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb_eta,
+                      double den, hipStream_t s);
+void launch_design_stats(Plan& P, const double* x_old, const double* x_new, const double* rho,
+                         hipStream_t s);
+
+// ---- host-side drivers (tk_solver.hip) ------------------------------------
+int solve(Plan& P, int prec, const double* rho, double penal, const double* F, double* U,
+          double rtol, int maxit, int flags, int* iters_out, double* relres_out, hipStream_t s);
+int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStream_t s);
+
+// ---- errors -----------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+}  // namespace tk
+
+#define TK_HIP(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::tk::set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
+      return 2;                                                                       \
+    }                                                                                 \
+  } while (0)
