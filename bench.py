@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Design iterations/s of the SIMP hot path at 256x128x128 H8 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (synthetic, no data files): the 256x128x128 unit-cube H8 cantilever
+-- all DOFs fixed on x = 0, a total unit force in -y spread over the nodes of
+the edge x = nelx, y = 0 -- volfrac 0.3, penal 3, rmin 3, eta 0.5, move 0.2,
+x0 = 0.3 with the protected strips imposed, beta = 1 (the first continuation
+stage).  A step is one full design iteration on the device: linear solve
+(matrix-free multigrid-PCG), energies + chain rule, OC bisection, re-filtering
+and the free-volume statistics, with the host reading the per-iteration
+scalars exactly as top3d does.
+
+The headline value is the mixed-precision path (FP32 operator and multigrid
+inside FP64 iterative refinement to a relative residual of 1e-8).  The same
+steps are then timed on the FP64 path at the same tolerance, and the
+compliance / sensitivity error of the mixed solve against an FP64 solve of the
+same design is reported.  The roofline object is for the dominant kernel
+class, timed with HIP events on the stream the library launches on.  The CPU
+baseline times the CPU oracle (assembled K + SuperLU, the reference's CPU
+path without PyPardiso) on a bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "design iters/s at 256x128x128 H8: mixed vs FP64 GPU, c err <=1e-6 vs FP64"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nelx", type=int, default=256)
+    ap.add_argument("--nely", type=int, default=128)
+    ap.add_argument("--nelz", type=int, default=128)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--operator-kind", type=int, default=0,
+                    help="0: element/sector kernel (default), 1: dense node kernel")
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def _log(args, *msg):
+    print("[bench]", *msg, file=sys.stderr, flush=True)
+
+
+def _setup(args):
+    import torch
+    from pytopo3d_amd.optimizer import DesignState, SIMPProblem
+    from pytopo3d_amd.problem import (edge_load_force_vector, element_stiffness,
+                                      protected_masks, support_node_flags)
+    nelx, nely, nelz = args.nelx, args.nely, args.nelz
+    F = edge_load_force_vector(nelx, nely, nelz)
+    solid, void, free = protected_masks(nelx, nely, nelz, None, None, None, None, F)
+    KE = element_stiffness(None, 1.0)
+    fixed = support_node_flags(nelx, nely, nelz, None)
+    problem = SIMPProblem(nelx, nely, nelz, 3.0, KE, F, fixed, solid, void,
+                          solver_options={"operator_kind": args.operator_kind})
+    x0 = np.full((nely, nelx, nelz), 0.3)
+    x0[solid] = 1.0
+    x0[void] = 0.0
+    target = 0.3 * int(free.sum())
+    return problem, x0, target, DesignState, torch
+
+
+def _run_path(problem, x0, target, DesignState, torch, precision, args):
+    """W untimed + K timed design iterations; returns (seconds, per-step records, state)."""
+    from pytopo3d_amd.optimizer import design_step
+    st = DesignState(problem, x0)
+    problem.have_U = False
+    beta, eta = 1.0, 0.5
+    problem.plan.filter_project(st.x, beta, eta, st.rf, st.rho, st.drho)
+    kw = dict(precision=precision, rtol=args.rtol, maxit=20000, solver="mg")
+    for i in range(args.warmup):
+        r = design_step(problem, st, 3.0, beta, eta, 0.2, target, kw)
+        _log(args, f"{precision} warmup {i}: c={r['c']:.6e} iters={r['solve_iters']} "
+                   f"oc={r['oc_steps']} solve={r['t_solve']*1e3:.1f}ms oc={r['t_oc']*1e3:.1f}ms")
+    torch.cuda.synchronize()
+    recs = []
+    c0 = problem.plan.counters()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = design_step(problem, st, 3.0, beta, eta, 0.2, target, kw)
+        recs.append(r)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    c1 = problem.plan.counters()
+    work = {k: c1[k] - c0[k] for k in c1}
+    for i, r in enumerate(recs):
+        _log(args, f"{precision} step {i}: c={r['c']:.6e} iters={r['solve_iters']} oc={r['oc_steps']} "
+                   f"solve={r['t_solve']*1e3:.1f}ms sens={r['t_sens']*1e3:.1f}ms oc={r['t_oc']*1e3:.1f}ms")
+    return dt, recs, st, work
+
+
+def _time_kernels(problem, st, torch, args):
+    """Average launch duration (HIP events on the library's stream) of the
+    kernel classes, with their algorithmic bytes and flops per launch."""
+    plan = problem.plan
+    nele, ndof = plan.nele, plan.ndof
+    dev = problem.dev
+    reps = args.kernel_reps
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    # operator, FP32 and FP64: the last solve of each precision left its scales
+    u32 = torch.randn(ndof, dtype=torch.float32, device=dev)
+    y32 = torch.empty_like(u32)
+    u64 = u32.double()
+    y64 = torch.empty_like(u64)
+    U = torch.zeros(ndof, dtype=torch.float64, device=dev)
+    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="mixed")
+    t = timed(lambda: plan.apply_K(None, 3.0, u32, y32, precision="mixed"))
+    out["operator_fp32"] = {"s": t, "bytes": 4 * (2 * ndof + nele), "flops": 1152 * nele,
+                            "kernel": "k_apply<float>"}
+    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="fp64")
+    t = timed(lambda: plan.apply_K(None, 3.0, u64, y64, precision="fp64"))
+    out["operator_fp64"] = {"s": t, "bytes": 8 * (2 * ndof + nele), "flops": 1152 * nele,
+                            "kernel": "k_apply<double>"}
+    rf, rho, d = problem.new_elem(), problem.new_elem(), problem.new_elem()
+    t = timed(lambda: plan.filter_project(st.x, 1.0, 0.5, rf, rho, d))
+    out["filter_project"] = {"s": t, "bytes": nele * (8 + 8 + 1 + 24),
+                             "flops": 2 * 93 * nele, "kernel": "k_filter_project"}
+    return out
+
+
+def _cpu_baseline(args):
+    """Oracle design iterations (assembled K + SuperLU + sparse filter + OC) on
+    this host at 30x15x10, scaled per element to the 256x128x128 workload."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    nelx, nely, nelz = 30, 15, 10
+    F = oracle.edge_load_force_vector(nelx, nely, nelz)
+    solid, void, free = oracle.boundary_protection(nelx, nely, nelz, None, None, None, None, F)
+    KE = oracle.element_stiffness(None, 1.0)
+    freedofs, _ = oracle.support_dofs(nelx, nely, nelz)
+    H, Hs = oracle.filter_matrix(nelx, nely, nelz, 3.0)
+    edof = oracle.element_dofs(nelx, nely, nelz)
+    shape = (nely, nelx, nelz)
+    x = np.full(shape, 0.3)
+    x[solid], x[void] = 1.0, 0.0
+    target = 0.3 * free.sum()
+    _, rho, d = oracle.physical_density(x, H, Hs, 1.0, 0.5, solid, void)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        U = oracle.solve_displacement(rho, 3.0, KE, F, freedofs, nelx, nely, nelz)
+        ce = oracle.element_energy(U, edof, KE).reshape(shape, order="F")
+        dc = -3.0 * (1 - 1e-9) * rho ** 2.0 * ce
+        dcdx = oracle.filter_chain_rule(dc, d, H, Hs)
+        dvdx = oracle.filter_chain_rule(free.astype(float), d, H, Hs)
+        x, _ = oracle.oc_projected(x, dcdx, dvdx, 0.2, target, H, Hs, 1.0, 0.5, free, solid, void)
+        _, rho, d = oracle.physical_density(x, H, Hs, 1.0, 0.5, solid, void)
+        n += 1
+        if time.perf_counter() - t0 >= args.cpu_seconds:
+            break
+    dt = time.perf_counter() - t0
+    rate_sample = n / dt
+    scale = (nelx * nely * nelz) / float(args.nelx * args.nely * args.nelz)
+    return {
+        "value": rate_sample * scale,
+        "unit": "design iters/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{n} oracle design iterations (SuperLU direct solve, single-threaded) at "
+                   f"{nelx}x{nely}x{nelz} in {dt:.1f} s = {rate_sample:.3f} iters/s, scaled "
+                   f"linearly by element count to {args.nelx}x{args.nely}x{args.nelz} "
+                   "(optimistic: a 3-D direct solve grows faster than linearly)"),
+    }
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "decomposition is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    problem, x0, target, DesignState, torch = _setup(args)
+    plan = problem.plan
+    _log(args, "levels", plan.levels(), "nfree", problem.nfree)
+
+    t_mixed, recs_m, st_m, work_m = _run_path(problem, x0, target, DesignState, torch, "mixed", args)
+    value = args.steps / t_mixed
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "design iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.nelx}x{args.nely}x{args.nelz} H8 cantilever (edge load), "
+                         "design iterations at beta=1, mixed FP32 multigrid-PCG in FP64 "
+                         f"iterative refinement to rtol {args.rtol:g}"),
+            "nelx": args.nelx, "nely": args.nely, "nelz": args.nelz,
+            "volfrac": 0.3, "penal": 3.0, "rmin": 3.0, "beta": 1.0, "eta": 0.5, "move": 0.2,
+            "solver_rtol": args.rtol, "levels": plan.levels(),
+        },
+        "solve_iters_mean": float(np.mean([r["solve_iters"] for r in recs_m])),
+        "oc_steps_mean": float(np.mean([r["oc_steps"] for r in recs_m])),
+        "work_timed_mixed": work_m,
+        "phase_ms_mixed": {k: 1e3 * float(np.mean([r[k] for r in recs_m]))
+                           for k in ("t_solve", "t_sens", "t_oc")},
+    }
+    if not args.no_fp64:
+        t64, recs_d, st_d, work_d = _run_path(problem, x0, target, DesignState, torch, "fp64", args)
+        result["fp64_value"] = args.steps / t64
+        result["speedup_vs_fp64"] = value / result["fp64_value"]
+        result["work_timed_fp64"] = work_d
+        result["phase_ms_fp64"] = {k: 1e3 * float(np.mean([r[k] for r in recs_d]))
+                                   for k in ("t_solve", "t_sens", "t_oc")}
+        # compliance / sensitivity error of the mixed solve on the mixed run's design
+        rho = st_m.rho
+        Ua = torch.zeros(plan.ndof, dtype=torch.float64, device=problem.dev)
+        Ub = torch.zeros_like(Ua)
+        plan.solve(rho, 3.0, problem.F, Ua, rtol=1e-11, precision="fp64")
+        plan.solve(rho, 3.0, problem.F, Ub, rtol=args.rtol, precision="mixed")
+        ga, gb = problem.new_elem(), problem.new_elem()
+        ca = plan.energy(Ua, rho, 3.0, drho=st_m.drho, dc_dx=ga)
+        cb = plan.energy(Ub, rho, 3.0, drho=st_m.drho, dc_dx=gb)
+        result["c_rel_err_vs_fp64"] = abs(cb - ca) / abs(ca)
+        result["dc_rel_err_vs_fp64"] = float((gb - ga).abs().max() / ga.abs().max())
+        result["c_first_step_rel_diff"] = abs(recs_m[0]["c"] - recs_d[0]["c"]) / abs(recs_d[0]["c"])
+    kern = _time_kernels(problem, st_m, torch, args)
+    dom = kern["operator_fp32"]
+    achieved = dom["bytes"] / dom["s"] / 1e9
+    result["roofline"] = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom["kernel"],
+        "launch_us": dom["s"] * 1e6,
+        "flop_frac_fp32": dom["flops"] / dom["s"] / 1e12 / FP32_PEAK_TFLOPS,
+    }
+    result["kernels"] = {k: {"kernel": v["kernel"], "us": v["s"] * 1e6,
+                             "GBps": v["bytes"] / v["s"] / 1e9,
+                             "hbm_frac": v["bytes"] / v["s"] / 1e9 / HBM_PEAK_GBS,
+                             "TFLOPs": v["flops"] / v["s"] / 1e12} for k, v in kern.items()}
+    if not args.no_cpu:
+        result["cpu_baseline"] = _cpu_baseline(args)
+    else:
+        result["cpu_baseline"] = None
+    problem.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

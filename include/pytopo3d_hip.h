@@ -55,9 +55,11 @@ typedef struct tk_options {
   double cheb_hi;         /* (1.0) */
   int max_levels;         /* multigrid levels (12) */
   int coarse_max_dof;     /* coarsest grid DOF limit, <= 192 (192) */
-  double inner_rtol;      /* mixed precision: inner FP32 tolerance (1e-3) */
+  double inner_rtol;      /* mixed precision: inner FP32 tolerance (1e-4) */
   int inner_maxit;        /* mixed precision: inner iteration cap per refinement step (200) */
   int check_every;        /* iterations between host convergence checks (8) */
+  int operator_kind;      /* 0: element kernel in the Walsh-Hadamard sector form when KE is
+                             mirror-symmetric (default), 1: dense node-gather kernel */
 } tk_options;
 
 /* Fill *opt with the defaults. */
@@ -107,7 +109,9 @@ int tk_filter_transpose2(tk_plan* plan, const double* g_c, const double* g_v,
 
 /* y = K(rho) u on the free DOFs (identity rows on fixed DOFs), K = sum_e
  * (1e-9 + (1-1e-9) rho_e^penal) KE.  prec TK_FP64: u, y double*; TK_MIXED_FP32:
- * float*.  Replaces the assembly + Kff of pytopo3d/core/optimizer.py:584-591. */
+ * float*.  rho == NULL reuses the element scales of the last tk_solve on this
+ * plan (a single kernel launch; used for timing).  Replaces the assembly + Kff
+ * of pytopo3d/core/optimizer.py:584-591. */
 int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const void* u, void* y,
                void* stream);
 /* diag(K(rho)) (1 on fixed DOFs), ndof doubles. */

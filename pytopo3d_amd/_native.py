@@ -42,6 +42,7 @@ class TkOptions(ctypes.Structure):
         ("inner_rtol", ctypes.c_double),
         ("inner_maxit", ctypes.c_int),
         ("check_every", ctypes.c_int),
+        ("operator_kind", ctypes.c_int),
     ]
 
 

@@ -65,7 +65,7 @@ static int dmalloc(T** p, size_t n) {
 
 static void free_plan(Plan* P) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
-  f(P->KE64); f(P->KE32); f(P->sten_off); f(P->sten_w); f(P->Hs); f(P->eflags);
+  f(P->KE64); f(P->KE32); f(P->sten_off); f(P->sten_w); f(P->Hs); f(P->eflags); f(P->fcols);
   for (int l = 0; l < kMaxLevels; ++l) {
     Level& L = P->lv[l];
     f(L.fixed);
@@ -132,6 +132,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->inner_rtol > 0.0) P->opt.inner_rtol = o->inner_rtol;
     if (o->inner_maxit > 0) P->opt.inner_maxit = o->inner_maxit;
     if (o->check_every > 0) P->opt.check_every = o->check_every;
+    if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
   }
   if (!(P->opt.cheb_lo < P->opt.cheb_hi)) {
     delete P;
@@ -153,6 +154,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   P->g = make_grid(nelx, nely, nelz);
   std::memcpy(P->KE, KE, sizeof(double) * 576);
   P->lam_bound = lambda_bound(KE);
+  operator_setup(*P);
+  P->use_el = P->sector_ok && P->opt.operator_kind == 0;
   P->rmin = rmin;
   // stencil: integer offsets with rmin - |d| > 1e-9, ordered (dz, dx, dy) ascending
   const int R = (int)std::ceil(rmin);
@@ -250,6 +253,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     free_plan(P);
     return 2;
   }
+  if (!filter_tiled_setup(*P, off, w)) P->ftiled = false;
   launch_filter_hs(*P, s);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -323,6 +327,7 @@ void tk_options_default(tk_options* o) {
   o->inner_rtol = d.inner_rtol;
   o->inner_maxit = d.inner_maxit;
   o->check_every = d.check_every;
+  o->operator_kind = d.operator_kind;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,
@@ -449,18 +454,23 @@ int tk_filter_transpose2(tk_plan* plan, const double* g_c, const double* g_v,
 int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const void* u, void* y,
                void* stream) {
   TK_PLAN_OR_FAIL(plan);
-  if (!rho || !u || !y || !(std::isfinite(penal) && penal > 0)) {
-    set_error("rho, u, y are required and penal must be positive");
+  if (!u || !y || (rho && !(std::isfinite(penal) && penal > 0))) {
+    set_error("u and y are required and penal must be positive");
     return 1;
   }
-  hipStream_t s = (hipStream_t)stream;
-  if (prec == TK_FP64) tk::launch_stiffness_scale<double>(P, rho, penal, P.lv[0].f64.s, s);
-  else if (prec == TK_MIXED_FP32) tk::launch_stiffness_scale<float>(P, rho, penal, P.lv[0].f32.s, s);
-  else {
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
     set_error("unknown precision");
     return 1;
   }
-  P.last_penal = -1.0;
+  hipStream_t s = (hipStream_t)stream;
+  if (rho) {
+    if (prec == TK_FP64) tk::launch_stiffness_scale<double>(P, rho, penal, P.lv[0].f64.s, s);
+    else tk::launch_stiffness_scale<float>(P, rho, penal, P.lv[0].f32.s, s);
+    P.last_penal = -1.0;
+  } else if (P.last_penal < 0.0 || (prec == TK_MIXED_FP32 && P.last_prec == TK_FP64)) {
+    set_error("rho == NULL reuses the scales of the last tk_solve in this precision, but there is none");
+    return 1;
+  }
   return tk::apply_fine(P, prec, u, y, s);
 }
 

@@ -113,20 +113,50 @@ __device__ __forceinline__ double load_coherent(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Linear block id and block count of a 1-, 2- or 3-D grid.
+__device__ __forceinline__ int block_id() {
+  return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+__device__ __forceinline__ int block_count() { return (int)(gridDim.x * gridDim.y * gridDim.z); }
+
+// Two-phase grid reductions (the fast path): every block writes its partial
+// value(s) to partials[i * G + block] with no atomics, and a one-block
+// finalize kernel (tk_reduce.hip) combines them in a fixed order and applies
+// the scalar update.  A single device-scope counter touched by every block
+// serialises at the memory side and costs tens of nanoseconds per block, so
+// the counter-based "last block" variant below is kept only for launches with
+// very few blocks.  All threads of the block must call (no early return).
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+    const int G = block_count(), b = block_id();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) partials[i * G + b] = v[i];
+  }
+}
+
+__device__ __forceinline__ void block_partial_max(double v, double* partials, int slot) {
+  v = block_max(v);
+  if (threadIdx.x == 0) partials[slot * block_count() + block_id()] = v;
+}
+
 // Grid-wide sum of NV values.  Every thread of every block must call it (no
 // early returns before it).  The last block to finish calls fin(total) from
-// thread 0 and re-arms the counter.  partials holds NV * gridDim.x doubles.
+// thread 0 and re-arms the counter.  partials holds NV * (number of blocks)
+// doubles.
 template <int NV, class Fin>
 __device__ __forceinline__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter,
                                          Fin fin) {
   block_sum<NV>(v);
   __shared__ int am_last;
+  const int G = block_count();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) partials[i * gridDim.x + blockIdx.x] = v[i];
+    for (int i = 0; i < NV; ++i) partials[i * G + block_id()] = v[i];
     __threadfence();
     const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == gridDim.x - 1);
+    am_last = (t == (unsigned)G - 1);
   }
   __syncthreads();
   if (am_last) {
@@ -134,9 +164,9 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* partials, unsi
     double s[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+    for (int b = threadIdx.x; b < G; b += blockDim.x)
 #pragma unroll
-      for (int i = 0; i < NV; ++i) s[i] += load_coherent(partials + i * gridDim.x + b);
+      for (int i = 0; i < NV; ++i) s[i] += load_coherent(partials + i * G + b);
     block_sum<NV>(s);
     if (threadIdx.x == 0) {
       *counter = 0u;
@@ -149,17 +179,18 @@ template <class Fin>
 __device__ __forceinline__ void grid_max(double v, double* partials, unsigned* counter, Fin fin) {
   v = block_max(v);
   __shared__ int am_last;
+  const int G = block_count();
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = v;
+    partials[block_id()] = v;
     __threadfence();
     const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == gridDim.x - 1);
+    am_last = (t == (unsigned)G - 1);
   }
   __syncthreads();
   if (am_last) {
     __threadfence();
     double s = -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+    for (int b = threadIdx.x; b < G; b += blockDim.x)
       s = fmax(s, load_coherent(partials + b));
     s = block_max(s);
     if (threadIdx.x == 0) {

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(
       wv[e] = ((fl[e] == 0 ? 1.0 : 0.0) * d) / h;
     }
   }
-  grid_sum<1>(part, partials, counter, [&](double (&t)[1]) { sc->c = t[0]; });
+  block_partials<1>(part, partials);
 }
 
 void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
@@ -62,6 +62,7 @@ void launch_energy(Plan& P, const double* U, const double* rho, const double* dr
   hipLaunchKernelGGL(k_energy, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.KE64, U, rho,
                      drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials, P.counter,
                      P.sc);
+  finalize(P, kFinC, nblocks(P.g.nele), s);
 }
 
 }  // namespace tk

@@ -91,9 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_free_sum_projected(
     project(f, beta, eta, tb_eta, den, 0, r, d);
     v[0] = r;
   }
-  grid_sum<1>(v, partials, counter, [&](double (&s)[1]) {
-    if (slot == 0) sc->sum0 = s[0]; else sc->sum1 = s[0];
-  });
+  block_partials<1>(v, partials);
 }
 
 // Attainable-volume designs (pytopo3d/core/optimizer.py:501-531):
@@ -114,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_count_free(int n, const uint8_t* __r
                                                         Scalars* sc) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {(e < n && fl[e] == 0) ? 1.0 : 0.0};
-  grid_sum<1>(v, partials, counter, [&](double (&s)[1]) { sc->sum0 = s[0]; });
+  block_partials<1>(v, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -126,6 +124,10 @@ void launch_filter_hs(Plan& P, hipStream_t s) {
 void launch_filter_project(Plan& P, const double* x, double beta, double eta, double tb_eta,
                            double den, double* rf, double* rho, double* drho, hipStream_t s) {
   P.cnt.filter_passes++;
+  if (P.ftiled) {
+    tiled_filter_project(P, x, beta, eta, tb_eta, den, rf, rho, drho, s);
+    return;
+  }
   hipLaunchKernelGGL(k_filter_project, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g,
                      P.nsten, P.sten_off, P.sten_w, P.Hs, P.eflags, x, beta, eta, tb_eta, den,
                      rf, rho, drho);
@@ -134,6 +136,11 @@ void launch_filter_project(Plan& P, const double* x, double beta, double eta, do
 void launch_filter_transpose2(Plan& P, const double* wc, const double* wv, double* oc,
                               double* ov, hipStream_t s) {
   P.cnt.filter_passes++;
+  if (P.ftiled) {
+    tiled_filter_plain(P, wc, oc, s);
+    if (wv) tiled_filter_plain(P, wv, ov, s);
+    return;
+  }
   hipLaunchKernelGGL(k_filter_transpose2, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g,
                      P.nsten, P.sten_off, P.sten_w, wc, wv, oc, ov);
 }
@@ -141,9 +148,14 @@ void launch_filter_transpose2(Plan& P, const double* wc, const double* wv, doubl
 void launch_free_sum_projected(Plan& P, const double* x, double beta, double eta, double tb_eta,
                                double den, int slot, hipStream_t s) {
   P.cnt.filter_passes++;
+  if (P.ftiled) {
+    tiled_free_sum(P, x, beta, eta, tb_eta, den, slot, s);
+    return;
+  }
   hipLaunchKernelGGL(k_free_sum_projected, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g,
                      P.nsten, P.sten_off, P.sten_w, P.Hs, P.eflags, x, beta, eta, tb_eta, den,
                      slot, P.partials, P.counter, P.sc);
+  finalize(P, slot == 0 ? kFinSum0 : kFinSum1, nblocks(P.g.nele), s);
 }
 
 void launch_bounds_design(Plan& P, int upper, double* out, hipStream_t s) {
@@ -154,6 +166,7 @@ void launch_bounds_design(Plan& P, int upper, double* out, hipStream_t s) {
 void launch_count_free(Plan& P, hipStream_t s) {
   hipLaunchKernelGGL(k_count_free, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
                      P.eflags, P.partials, P.counter, P.sc);
+  finalize(P, kFinSum0, nblocks(P.g.nele), s);
 }
 
 }  // namespace tk

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_ratio(int n, const uint8_t* __res
     ratio[e] = r;
     m = r;
   }
-  grid_max(m, partials, counter, [&](double v) { sc->ratio_scale = v; });
+  block_partial_max(m, partials, 0);
 }
 
 __global__ __launch_bounds__(kBlock) void k_oc_normalize(int n, const uint8_t* __restrict__ fl,
@@ -104,21 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_volume(Grid g, int nsten, const i
     project(f, beta, eta, tb_eta, den, 0, r, d);
     v[0] = r;
   }
-  grid_sum<1>(v, partials, counter, [&](double (&t)[1]) {
-    const double V = t[0];
-    const double lam = sc->lam;
-    sc->V = V;
-    const double err = fabs(V - sc->target);
-    if (err < sc->best_err) {
-      sc->best_err = err;
-      sc->best_lam = lam;
-    }
-    if (V > sc->target) sc->lo = lam;
-    else sc->hi = lam;
-    sc->steps += 1;
-    const double rel = (sc->hi - sc->lo) / fmax(sc->hi + sc->lo, DBL_MIN);
-    if (rel < sc->tol || sc->steps >= sc->max_steps) sc->oc_done = 1;
-  });
+  block_partials<1>(v, partials);
 }
 
 // change = max |x_new - x| (free), vol_sum = sum rho (free), gray = #(0.05 < rho < 0.95) (free)
@@ -137,37 +123,13 @@ __global__ __launch_bounds__(kBlock) void k_design_stats(int n, const uint8_t* _
     v[0] = r;
     v[1] = (r > 0.05 && r < 0.95) ? 1.0 : 0.0;
   }
-  // block level
   block_sum<2>(v);
   m = block_max(m);
-  __shared__ int am_last;
-  const int G = gridDim.x;
   if (threadIdx.x == 0) {
+    const int G = gridDim.x;
     partials[blockIdx.x] = v[0];
     partials[G + blockIdx.x] = v[1];
     partials[2 * G + blockIdx.x] = m;
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == (unsigned)G - 1);
-  }
-  __syncthreads();
-  if (am_last) {
-    __threadfence();
-    double s[2] = {0.0, 0.0};
-    double mm = -INFINITY;
-    for (int b = threadIdx.x; b < G; b += blockDim.x) {
-      s[0] += load_coherent(partials + b);
-      s[1] += load_coherent(partials + G + b);
-      mm = fmax(mm, load_coherent(partials + 2 * G + b));
-    }
-    block_sum<2>(s);
-    mm = block_max(mm);
-    if (threadIdx.x == 0) {
-      *counter = 0u;
-      sc->vol_sum = s[0];
-      sc->gray_cnt = s[1];
-      sc->change = mm;
-    }
   }
 }
 
@@ -175,6 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_design_stats(int n, const uint8_t* _
 void launch_oc_ratio(Plan& P, const double* dc, const double* dv, double* ratio, hipStream_t s) {
   hipLaunchKernelGGL(k_oc_ratio, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele, P.eflags,
                      dc, dv, ratio, P.partials, P.counter, P.sc);
+  finalize(P, kFinRatioMax, nblocks(P.g.nele), s);
 }
 
 void launch_oc_normalize(Plan& P, double* ratio, hipStream_t s) {
@@ -195,15 +158,21 @@ void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double m
 void launch_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb_eta,
                       double den, hipStream_t s) {
   P.cnt.oc_steps++;
+  if (P.ftiled) {
+    tiled_oc_volume(P, cand, beta, eta, tb_eta, den, s);
+    return;
+  }
   hipLaunchKernelGGL(k_oc_volume, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.nsten,
                      P.sten_off, P.sten_w, P.Hs, P.eflags, cand, beta, eta, tb_eta, den,
                      P.partials, P.counter, P.sc);
+  finalize(P, kFinOcVolume, nblocks(P.g.nele), s);
 }
 
 void launch_design_stats(Plan& P, const double* x_old, const double* x_new, const double* rho,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_design_stats, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
                      P.eflags, x_old, x_new, rho, P.partials, P.counter, P.sc);
+  finalize(P, kFinStats, nblocks(P.g.nele), s);
 }
 
 }  // namespace tk

@@ -96,18 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(Grid g, const T* __restrict__ 
       dot[0] += (double)uc * (double)yc;
     }
   }
-  if (with_dot) {
-    grid_sum<1>(dot, partials, counter, [&](double (&t)[1]) {
-      sc->pq = t[0];
-      if (!(t[0] > 0.0)) {   // breakdown (or NaN): stop the iteration
-        sc->alpha = 0.0;
-        sc->done = 1;
-        sc->stalled = 1;
-      } else {
-        sc->alpha = sc->rz / t[0];
-      }
-    });
-  }
+  if (with_dot) block_partials<1>(dot, partials);
 }
 
 // r = b - K x
@@ -496,9 +485,14 @@ void launch_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, double
   if (&L == &P.lv[0]) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
   }
+  if (P.use_el) {
+    el_apply<T>(P, L, sl, u, y, with_dot, with_dot ? done_ptr(P) : nullptr, s);
+    return;
+  }
   hipLaunchKernelGGL(k_apply<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g, ke_of<T>(P), sl,
                      L.fixed, u, y, with_dot, with_dot ? done_ptr(P) : nullptr, P.partials,
                      P.counter, P.sc);
+  if (with_dot) finalize(P, kFinAlpha, nblocks(L.g.nnode), s);
 }
 
 template <typename T>
@@ -506,6 +500,10 @@ void launch_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* 
                      hipStream_t s, const int* done) {
   if (&L == &P.lv[0]) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
+  }
+  if (P.use_el) {
+    el_residual<T>(P, L, sl, b, x, r, done, s);
+    return;
   }
   hipLaunchKernelGGL(k_residual<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g, ke_of<T>(P),
                      sl, L.fixed, b, x, r, done);
@@ -517,6 +515,10 @@ void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, cons
   if (&L == &P.lv[0] && !x_zero) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
   }
+  if (P.use_el && !x_zero) {
+    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, c0, 0.0, 0.0, 1, done_ptr(P), s);
+    return;
+  }
   hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g,
                      ke_of<T>(P), sl, L.fixed, dinv, b, x, xo, d, (T)c0, x_zero, done_ptr(P));
 }
@@ -526,6 +528,10 @@ void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const
                       T* xo, T* d, double c1, double c2, hipStream_t s) {
   if (&L == &P.lv[0]) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
+  }
+  if (P.use_el) {
+    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, 0.0, c1, c2, 0, done_ptr(P), s);
+    return;
   }
   hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g,
                      ke_of<T>(P), sl, L.fixed, dinv, b, x, xo, d, (T)c1, (T)c2, done_ptr(P));

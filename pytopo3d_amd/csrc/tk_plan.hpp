@@ -34,9 +34,10 @@ struct Options {
   double cheb_hi = 1.0;
   int max_levels = kMaxLevels;
   int coarse_max_dof = kCoarseMaxDof;
-  double inner_rtol = 1e-3;      // mixed precision: inner FP32 solve tolerance
+  double inner_rtol = 1e-4;      // mixed precision: inner FP32 solve tolerance
   int inner_maxit = 200;
   int check_every = 8;           // Krylov convergence is read back every N iterations
+  int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node kernel
 };
 
 struct Counters {
@@ -51,12 +52,20 @@ struct Plan {
   double* KE64 = nullptr;         // device
   float* KE32 = nullptr;
   double lam_bound = 0.0;         // lambda_max(diag(KE)^-1 KE): bound of lambda_max(D^-1 K)
+  bool sector_ok = false;         // KE block-diagonalises in the Walsh-Hadamard basis
+  bool use_el = false;            // element kernel in use
+  double sec64[8][9];             // sector blocks (W^T KE W / 64)
+  float sec32[8][9];
   // density filter
   double rmin = 0.0;
   int nsten = 0, R = 0;
   int* sten_off = nullptr;        // packed (dy+16) | (dx+16)<<8 | (dz+16)<<16
   double* sten_w = nullptr;
   double* Hs = nullptr;
+  void* fcols = nullptr;          // column table of the LDS-tiled filter
+  int fhalo = 0;                  // its halo (largest stencil offset)
+  size_t flds = 0;                // its dynamic LDS bytes
+  bool ftiled = false;            // tiled filter available (halo <= 3)
   uint8_t* eflags = nullptr;      // per element, kSolid | kVoid
   long long nfree = 0;
   bool masks_set = false;
@@ -85,8 +94,22 @@ struct Plan {
   Counters cnt;
   // bookkeeping of the last solve
   double last_penal = -1.0;
+  int last_prec = 0;
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
 };
+
+// ---- two-phase reductions (tk_reduce.hip) -----------------------------------
+enum FinOp {
+  kFinSum0 = 0, kFinSum1, kFinNorm0, kFinNorm1, kFinC,
+  kFinAlpha, kFinNorms, kFinUpdate, kFinRz, kFinRzFlex,
+  kFinRatioMax, kFinOcVolume, kFinStats
+};
+constexpr int kRedBlocks = 2048;   // grid-stride reduction kernels use at most this many blocks
+void finalize(Plan& P, int op, int G, hipStream_t s, double a0 = 0.0, int i0 = 0);
+inline int red_blocks(long long n) {
+  const int b = nblocks(n);
+  return b < kRedBlocks ? (b > 0 ? b : 1) : kRedBlocks;
+}
 
 // ---- launchers (defined in the .hip files) --------------------------------
 // filter
@@ -126,25 +149,27 @@ template <typename T>
 void launch_prolong_add(const Level& c, const Level& f, const T* xc, T* xf, hipStream_t s,
                         const int* done);
 hipError_t coarse_setup_attr(size_t lds);
+bool operator_setup(Plan& P);
+bool filter_tiled_setup(Plan& P, const std::vector<int>& off, const std::vector<double>& w);
+void tiled_filter_project(Plan& P, const double* x, double beta, double eta, double tb, double den,
+                          double* rf, double* rho, double* drho, hipStream_t s);
+void tiled_filter_plain(Plan& P, const double* v, double* out, hipStream_t s);
+void tiled_free_sum(Plan& P, const double* x, double beta, double eta, double tb, double den,
+                    int slot, hipStream_t s);
+void tiled_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb, double den,
+                     hipStream_t s);
+template <typename T>
+void el_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, int with_dot,
+              const int* done, hipStream_t s);
+template <typename T>
+void el_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T* r,
+                 const int* done, hipStream_t s);
+template <typename T>
+void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x, T* xo,
+             T* d, double c0, double c1, double c2, int first, const int* done, hipStream_t s);
 void launch_coarse_setup(Plan& P, hipStream_t s);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);
-// Krylov vector kernels
-template <typename T>
-void launch_cg_init(Plan& P, const T* b, T* r, T* x, int x_zero, hipStream_t s);
-template <typename T>
-void launch_cg_update(Plan& P, T* x, T* r, const T* p, const T* q, hipStream_t s);
-template <typename T>
-void launch_cg_rz(Plan& P, const T* r, const T* z, int first, hipStream_t s);
-template <typename T>
-void launch_cg_dir(Plan& P, const T* z, T* p, int first, hipStream_t s);
-template <typename T>
-void launch_jacobi(Plan& P, const T* dinv, const T* r, T* z, hipStream_t s);
-void launch_ir_residual(Plan& P, const double* F, const double* x, double* r64, float* r32,
-                        hipStream_t s);
-void launch_ir_scale_r(Plan& P, const double* r64, float* r32, hipStream_t s);
-void launch_ir_correct(Plan& P, double* x, const float* d, hipStream_t s);
-void launch_norm2(Plan& P, const double* v, int n, int slot, hipStream_t s);
 // energy
 void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
                    double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s);

@@ -10,6 +10,7 @@
 #include "tk_plan.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <utility>
 
 namespace tk {
@@ -30,78 +31,48 @@ __global__ __launch_bounds__(kBlock) void k_copy_zero(int n, const T* __restrict
   x[i] = T(0);
 }
 
-// sum (a.a) and (b.b) -> sc->rr, sc->bb; arms the iteration.
+// b.b and r.r partials (kFinNorms arms the iteration)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_norms(int n, const T* __restrict__ b,
-                                                      const T* __restrict__ r, double tol2,
-                                                      int maxit, double* partials,
-                                                      unsigned* counter, Scalars* sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                      const T* __restrict__ r, double* partials) {
   double v[2] = {0.0, 0.0};
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double bi = b[i], ri = r[i];
-    v[0] = bi * bi;
-    v[1] = ri * ri;
+    v[0] += bi * bi;
+    v[1] += ri * ri;
   }
-  grid_sum<2>(v, partials, counter, [&](double (&t)[2]) {
-    sc->bb = t[0];
-    sc->rr = t[1];
-    sc->rr0 = t[1];
-    sc->tol2 = tol2;
-    sc->maxit = maxit;
-    sc->iter = 0;
-    sc->stalled = 0;
-    sc->done = (t[1] <= tol2 * t[0]) ? 1 : 0;
-    if (!(t[1] == t[1])) { sc->done = 1; sc->stalled = 1; }
-  });
+  block_partials<2>(v, partials);
 }
 
-// x += alpha p; r -= alpha q; rr -> convergence test.
+// x += alpha p; r -= alpha q; r.r partials (kFinUpdate tests convergence)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const int* __restrict__ done,
-                                                       double* partials, unsigned* counter,
-                                                       Scalars* sc) {
+                                                       double* partials, const Scalars* sc) {
   if (*done) return;
   const T alpha = (T)sc->alpha;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {0.0};
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
-    v[0] = (double)ri * (double)ri;
+    v[0] += (double)ri * (double)ri;
   }
-  grid_sum<1>(v, partials, counter, [&](double (&t)[1]) {
-    sc->rr = t[0];
-    sc->iter += 1;
-    if (t[0] <= sc->tol2 * sc->bb) sc->done = 1;
-    if (sc->iter >= sc->maxit) sc->done = 1;
-    if (!(t[0] == t[0])) { sc->done = 1; sc->stalled = 1; }
-  });
+  block_partials<1>(v, partials);
 }
 
-// rz = r.z; beta = rz / rz_old (first: beta = 0).
+// r.z partials (kFinRz computes beta)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_rz(int n, const T* __restrict__ r,
-                                                   const T* __restrict__ z, int first,
-                                                   const int* __restrict__ done, double* partials,
-                                                   unsigned* counter, Scalars* sc) {
+                                                   const T* __restrict__ z,
+                                                   const int* __restrict__ done, double* partials) {
   if (*done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {0.0};
-  if (i < n) v[0] = (double)r[i] * (double)z[i];
-  grid_sum<1>(v, partials, counter, [&](double (&t)[1]) {
-    if (first) {
-      sc->beta = 0.0;
-    } else {
-      sc->beta = t[0] / sc->rz;
-    }
-    sc->rz = t[0];
-    if (!(t[0] > 0.0)) { sc->done = 1; sc->stalled = (t[0] == 0.0) ? 0 : 1; }
-  });
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v[0] += (double)r[i] * (double)z[i];
+  block_partials<1>(v, partials);
 }
 
 template <typename T>
@@ -123,37 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const T* __restrict__ 
   if (done && *done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) z[i] = dinv[i] * r[i];
-}
-
-// r32 = r64 / ||r64||  (norm taken from sc->rr)
-__global__ __launch_bounds__(kBlock) void k_ir_scale_r(int n, const double* __restrict__ r64,
-                                                        float* __restrict__ r32,
-                                                        const Scalars* __restrict__ sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double inv = 1.0 / sqrt(sc->norm0);
-  r32[i] = (float)(r64[i] * inv);
-}
-
-// x64 += ||r|| d32
-__global__ __launch_bounds__(kBlock) void k_ir_correct(int n, double* __restrict__ x,
-                                                        const float* __restrict__ d,
-                                                        const Scalars* __restrict__ sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  x[i] += sqrt(sc->norm0) * (double)d[i];
-}
-
-// ||v||^2 -> norm{slot}
-__global__ __launch_bounds__(kBlock) void k_norm2(int n, const double* __restrict__ v, int slot,
-                                                   double* partials, unsigned* counter,
-                                                   Scalars* sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double t[1] = {0.0};
-  if (i < n) t[0] = v[i] * v[i];
-  grid_sum<1>(t, partials, counter, [&](double (&s)[1]) {
-    if (slot == 0) sc->norm0 = s[0]; else sc->norm1 = s[0];
-  });
 }
 
 // F with fixed DOFs zeroed (the reduced system's right-hand side).
@@ -181,6 +121,51 @@ __global__ __launch_bounds__(kBlock) void k_zero_fixed(int nnode, const uint8_t*
 #pragma unroll
   for (int c = 0; c < 3; ++c)
     if ((f >> c) & 1) U[3 * n + c] = 0.0;
+}
+
+// ---- mixed precision: FP64 flexible CG around an FP32 multigrid correction ----
+// r32 = r64 / ||r64||  (||r||^2 = sc->rr, the current CG residual)
+__global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __restrict__ r64,
+                                                         float* __restrict__ r32,
+                                                         const int* __restrict__ done,
+                                                         const Scalars* __restrict__ sc) {
+  if (*done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rr = sc->rr;
+  const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
+  r32[i] = (float)(r64[i] * inv);
+}
+
+// z = ||r|| z32; partials of r.z and r.z_old (kFinRzFlex: Polak-Ribiere beta)
+__global__ __launch_bounds__(kBlock) void k_cg_rz_flex(int n, const double* __restrict__ r,
+                                                        const float* __restrict__ z32,
+                                                        const double* __restrict__ zold,
+                                                        int first, const int* __restrict__ done,
+                                                        double* partials, const Scalars* sc) {
+  if (*done) return;
+  const double nrm = sqrt(sc->rr);
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double z = nrm * (double)z32[i];
+    v[0] += r[i] * z;
+    if (!first) v[1] += r[i] * zold[i];
+  }
+  block_partials<2>(v, partials);
+}
+
+// p = z + beta p, z_old = z
+__global__ __launch_bounds__(kBlock) void k_cg_dir_flex(int n, const float* __restrict__ z32,
+                                                         double* __restrict__ zold,
+                                                         double* __restrict__ p, int first,
+                                                         const int* __restrict__ done,
+                                                         const Scalars* __restrict__ sc) {
+  if (*done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double z = sqrt(sc->rr) * (double)z32[i];
+  p[i] = first ? z : z + sc->beta * p[i];
+  zold[i] = z;
 }
 
 // ------------------------------------------------------------------ helpers
@@ -277,12 +262,12 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   } else {
     launch_residual<T>(P, L0, B0.s, b, x, r, s, nullptr);
   }
-  hipLaunchKernelGGL(k_cg_norms<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, b, r, rtol * rtol,
-                     maxit, P.partials, P.counter, P.sc);
+  hipLaunchKernelGGL(k_cg_norms<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, b, r, P.partials);
+  finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
   // z buffer: q is free until the first operator application
   const T* z = precondition<T>(P, r, q, jacobi, s);
-  hipLaunchKernelGGL(k_cg_rz<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, z, 1, done,
-                     P.partials, P.counter, P.sc);
+  hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z, done, P.partials);
+  finalize(P, kFinRz, red_blocks(n), s, 0.0, 1);
   hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, p, 1, done, P.sc);
   int it = 0;
   const int chunk = P.opt.check_every > 0 ? P.opt.check_every : 8;
@@ -290,12 +275,14 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
     const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
     for (int k = 0; k < m; ++k) {
       launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, s);
-      hipLaunchKernelGGL(k_cg_update<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, x, r, p, q, done,
-                         P.partials, P.counter, P.sc);
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p, q,
+                         done, P.partials, P.sc);
+      finalize(P, kFinUpdate, red_blocks(n), s);
       // the preconditioner output must not alias q (read by nothing after the update)
       const T* zz = precondition<T>(P, r, q, jacobi, s);
-      hipLaunchKernelGGL(k_cg_rz<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, zz, 0, done,
-                         P.partials, P.counter, P.sc);
+      hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, zz, done,
+                         P.partials);
+      finalize(P, kFinRz, red_blocks(n), s, 0.0, 0);
       hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, zz, p, 0, done,
                          P.sc);
     }
@@ -305,6 +292,54 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   }
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in pcg");
+    return 2;
+  }
+  return 0;
+}
+
+// FP64 flexible PCG whose preconditioner is the FP32 V-cycle (or FP32 Jacobi)
+// applied to the normalised FP64 residual: every correction is computed in
+// FP32, while the residual recurrence, dot products and step lengths stay FP64,
+// so the iteration reaches FP64 accuracy whatever the stiffness contrast.
+static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rtol, int maxit,
+                     int jacobi, hipStream_t s) {
+  const int n = P.g.ndof;
+  Level& L0 = P.lv[0];
+  const int* done = &P.sc->done;
+  double *r = P.r64, *p = P.p64, *q = P.q64, *zold = P.x64;
+  float* r32 = L0.f32.b;
+  if (x_zero) hipLaunchKernelGGL(k_copy_zero<double>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, b, r, x);
+  else launch_residual<double>(P, L0, L0.f64.s, b, x, r, s, nullptr);
+  hipLaunchKernelGGL(k_cg_norms<double>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, b, r,
+                     P.partials);
+  finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
+  auto correction = [&](int first) {
+    hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done, P.sc);
+    const float* z32 = precondition<float>(P, r32, P.q32, jacobi, s);
+    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z32, zold,
+                       first, done, P.partials, P.sc);
+    finalize(P, kFinRzFlex, red_blocks(n), s, 0.0, first);
+    hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z32, zold, p, first,
+                       done, P.sc);
+  };
+  correction(1);
+  int it = 0;
+  const int chunk = P.opt.check_every > 0 ? P.opt.check_every : 8;
+  while (it < maxit) {
+    const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
+    for (int k = 0; k < m; ++k) {
+      launch_apply<double>(P, L0, L0.f64.s, p, q, nullptr, 1, s);
+      hipLaunchKernelGGL(k_cg_update<double>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p,
+                         q, done, P.partials, P.sc);
+      finalize(P, kFinUpdate, red_blocks(n), s);
+      correction(0);
+    }
+    it += m;
+    if (read_scalars(P, s)) return 2;
+    if (P.sc_host->done) break;
+  }
+  if (hipGetLastError() != hipSuccess) {
+    set_error("kernel launch failed in mixed pcg");
     return 2;
   }
   return 0;
@@ -328,6 +363,7 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   launch_coarse_setup(P, s);
   TK_HIP(hipGetLastError());
   P.last_penal = penal;
+  P.last_prec = prec;
   return 0;
 }
 
@@ -364,48 +400,20 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
       return 3;
     }
   } else {
-    // FP64 iterative refinement around an FP32 PCG solve.
-    if (!use_x0) hipLaunchKernelGGL(k_fill<double>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, U, 0.0);
-    hipLaunchKernelGGL(k_norm2, dim3(nblocks(n)), dim3(kBlock), 0, s, n, b, 1, P.partials,
-                       P.counter, P.sc);
-    int outer = 0;
-    const int max_outer = 60;
-    for (;;) {
-      launch_residual<double>(P, P.lv[0], P.lv[0].f64.s, b, U, P.r64, s, nullptr);
-      hipLaunchKernelGGL(k_norm2, dim3(nblocks(n)), dim3(kBlock), 0, s, n, P.r64, 0, P.partials,
-                         P.counter, P.sc);
-      if (read_scalars(P, s)) return 2;
-      const double bb = P.sc_host->norm1, rr = P.sc_host->norm0;
-      relres = (bb > 0.0) ? std::sqrt(rr / bb) : 0.0;
-      if (!(relres == relres)) {
-        set_error("CG failed to converge: non-finite residual in refinement");
-        return 3;
-      }
-      if (relres <= rtol || rr == 0.0 || outer >= max_outer || total_it >= maxit) break;
-      hipLaunchKernelGGL(k_ir_scale_r, dim3(nblocks(n)), dim3(kBlock), 0, s, n, P.r64,
-                         P.lv[0].f32.b, P.sc);
-      // inner tolerance: no tighter than needed for the outer target
-      double in_tol = P.opt.inner_rtol;
-      const double need = 0.5 * rtol / relres;
-      if (need > in_tol) in_tol = need;
-      if (in_tol > 0.5) in_tol = 0.5;
-      int inner_max = P.opt.inner_maxit;
-      if (inner_max > maxit - total_it) inner_max = maxit - total_it;
-      int rc = pcg<float>(P, P.lv[0].f32.b, P.x32, P.r32, P.p32, P.q32, 1, in_tol, inner_max,
-                          jacobi, s);
-      if (rc) return rc;
-      total_it += P.sc_host->iter;
-      hipLaunchKernelGGL(k_ir_correct, dim3(nblocks(n)), dim3(kBlock), 0, s, n, U, P.x32, P.sc);
-      ++outer;
-      P.cnt.ir_steps++;
-    }
+    int rc = pcg_mixed(P, b, U, !use_x0, rtol, maxit, jacobi, s);
+    if (rc) return rc;
+    total_it = P.sc_host->iter;
+    relres = (P.sc_host->bb > 0.0) ? std::sqrt(P.sc_host->rr / P.sc_host->bb) : 0.0;
+    P.cnt.ir_steps += total_it;
     P.cnt.cg_iters += total_it;
   }
   if (iters_out) *iters_out = total_it;
   if (relres_out) *relres_out = relres;
   if (!(relres <= rtol)) {
-    set_error("CG failed to converge: relative residual " + std::to_string(relres) + " after " +
-              std::to_string(total_it) + " iterations");
+    char buf[160];
+    snprintf(buf, sizeof(buf), "CG failed to converge: relative residual %.3e (target %.3e) after %d iterations",
+             relres, rtol, total_it);
+    set_error(buf);
     return 3;
   }
   TK_HIP(hipGetLastError());

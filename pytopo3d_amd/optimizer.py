@@ -113,6 +113,40 @@ class SIMPProblem:
         self.plan.close()
 
 
+class DesignState:
+    """Device buffers of one design trajectory (element order)."""
+
+    def __init__(self, problem: SIMPProblem, x0: np.ndarray):
+        self.x = _to_dev_elem(x0, problem.dev)
+        self.x_new = problem.new_elem()
+        self.rf, self.rho, self.drho = problem.new_elem(), problem.new_elem(), problem.new_elem()
+        self.dc_dx, self.dv_dx = problem.new_elem(), problem.new_elem()
+
+
+def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float, eta: float,
+                move: float, target: float, solve_kw: dict) -> dict:
+    """One design iteration of pytopo3d/core/optimizer.py:578-696 on the device:
+    solve at the current rho, energies and the chain rule, the OC bisection,
+    re-filtering of the new design and the free-element statistics.  The
+    state's rho / drho must belong to st.x (as after filter_project)."""
+    plan = problem.plan
+    t0 = time.perf_counter()
+    U = problem.solve(st.rho, penal, **solve_kw)
+    t1 = time.perf_counter()
+    c = problem.sensitivities(U, st.rho, st.drho, penal, dc_dx=st.dc_dx, dv_dx=st.dv_dx)
+    t2 = time.perf_counter()
+    plan.oc_update(st.x, st.dc_dx, st.dv_dx, move, target, beta, eta, st.x_new)
+    plan.filter_project(st.x_new, beta, eta, st.rf, st.rho, st.drho)
+    plan.design_stats(st.rho, st.x, st.x_new)
+    res = plan.results()
+    t3 = time.perf_counter()
+    st.x, st.x_new = st.x_new, st.x
+    return {"c": c, "change": float(res["change"]), "vol": res["vol_sum"] / problem.nfree,
+            "gray": res["gray_cnt"] / problem.nfree, "oc_steps": int(res["oc_steps"]),
+            "solve_iters": problem.last_iters, "relres": problem.last_relres,
+            "t_solve": t1 - t0, "t_sens": t2 - t1, "t_oc": t3 - t2}
+
+
 def _make_problem(nelx, nely, nelz, rmin, material_params, elem_size, force_field, support_mask,
                   solid, void, force_vector, solver_options) -> SIMPProblem:
     KE = element_stiffness(material_params, elem_size)
@@ -294,10 +328,7 @@ def top3d(
     x0 = np.full(expected_shape, float(volfrac))
     x0[solid] = 1.0
     x0[void] = 0.0
-    x = _to_dev_elem(x0, dev)
-    x_new = problem.new_elem()
-    rf, rho, drho = problem.new_elem(), problem.new_elem(), problem.new_elem()
-    dc_dx, dv_dx = problem.new_elem(), problem.new_elem()
+    st = DesignState(problem, x0)
     t_setup = time.perf_counter() - t_setup
 
     tm = {"setup_s": t_setup, "solve_s": 0.0, "sens_s": 0.0, "oc_s": 0.0, "iter_s": 0.0,
@@ -311,7 +342,7 @@ def top3d(
         change = float("inf")
         stage_loop = 0
         c_prev = float("nan")
-        plan.filter_project(x, beta, eta, rf, rho, drho)
+        plan.filter_project(st.x, beta, eta, st.rf, st.rho, st.drho)
         vmin, vmax = plan.volume_bounds(beta, eta)
         target_within = vmin - 1.0e-10 <= volfrac <= vmax + 1.0e-10
         if volfrac < vmin - 1.0e-10:
@@ -334,25 +365,15 @@ def top3d(
             t0 = time.perf_counter()
             snapshot = save_history and ((total_loop - 1) % history_frequency == 0
                                          or stage_loop == maxloop)
-            rho_fea = rho.clone() if save_history else None
-            ts = time.perf_counter()
-            U = problem.solve(rho, penal, **solve_kw)
-            tm["solve_s"] += time.perf_counter() - ts
-            tm["solve_iters"].append(problem.last_iters)
-            ts = time.perf_counter()
-            c = problem.sensitivities(U, rho, drho, penal, dc_dx=dc_dx, dv_dx=dv_dx)
-            tm["sens_s"] += time.perf_counter() - ts
-            ts = time.perf_counter()
-            plan.oc_update(x, dc_dx, dv_dx, move, target_free_volume, beta, eta, x_new)
-            plan.filter_project(x_new, beta, eta, rf, rho, drho)
-            plan.design_stats(rho, x, x_new)
-            res = plan.results()
-            tm["oc_s"] += time.perf_counter() - ts
-            tm["oc_steps"].append(int(res["oc_steps"]))
-            change = float(res["change"])
-            vol_frac = res["vol_sum"] / problem.nfree
-            gray = res["gray_cnt"] / problem.nfree
-            x, x_new = x_new, x
+            rho_fea = st.rho.clone() if save_history else None
+            r = design_step(problem, st, penal, beta, eta, move, target_free_volume, solve_kw)
+            c = r["c"]
+            change, vol_frac, gray = r["change"], r["vol"], r["gray"]
+            tm["solve_s"] += r["t_solve"]
+            tm["sens_s"] += r["t_sens"]
+            tm["oc_s"] += r["t_oc"]
+            tm["solve_iters"].append(r["solve_iters"])
+            tm["oc_steps"].append(r["oc_steps"])
             c_delta = c - c_prev if np.isfinite(c_prev) else float("nan")
             c_prev = c
             elapsed = time.perf_counter() - t0
@@ -385,10 +406,10 @@ def top3d(
         })
 
     final_beta = beta_schedule[-1]
-    plan.filter_project(x, final_beta, eta, rf, rho, drho)
-    final_x = _to_host_elem(x, expected_shape)
-    final_rf = _to_host_elem(rf, expected_shape)
-    final_xPhys = _to_host_elem(rho, expected_shape)
+    plan.filter_project(st.x, final_beta, eta, st.rf, st.rho, st.drho)
+    final_x = _to_host_elem(st.x, expected_shape)
+    final_rf = _to_host_elem(st.rf, expected_shape)
+    final_xPhys = _to_host_elem(st.rho, expected_shape)
     if not final_evaluation:
         problem.close()
         if timings is not None:

@@ -103,14 +103,18 @@ def _free_operator(rho, penal, KE, nelx, nely, nelz, support_mask=None):
     return K, free, fixed
 
 
-@pytest.mark.parametrize("nelx,nely,nelz", [(3, 2, 2), (6, 4, 3), (9, 7, 5)])
-def test_apply_K_matches_assembled_matrix(nelx, nely, nelz):
-    note("apply_K", nelx, nely, nelz)
+OP_GRIDS = [(3, 2, 2), (6, 4, 3), (9, 7, 5), (17, 16, 9), (31, 15, 12), (20, 33, 18)]
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("nelx,nely,nelz", OP_GRIDS)
+def test_apply_K_matches_assembled_matrix(nelx, nely, nelz, kind):
+    note("apply_K", nelx, nely, nelz, "operator_kind", kind)
     rng = np.random.default_rng(3)
     shape = (nely, nelx, nelz)
     sm = np.zeros(shape, bool)
     sm[rng.integers(0, nely), 0, :] = True        # a support strip, not the default face
-    P, KE = make_plan(nelx, nely, nelz, 1.5, support_mask=sm)
+    P, KE = make_plan(nelx, nely, nelz, 1.5, support_mask=sm, options={"operator_kind": kind})
     rho = rng.uniform(0.01, 1.0, shape)
     K, free, fixed = _free_operator(rho, 3.0, KE, nelx, nely, nelz, sm)
     u = rng.standard_normal(3 * (nelx + 1) * (nely + 1) * (nelz + 1))
@@ -137,22 +141,23 @@ def test_apply_K_matches_assembled_matrix(nelx, nely, nelz):
     P.close()
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("solver", ["mg", "jacobi"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("nelx,nely,nelz", [(6, 4, 3), (12, 6, 5), (16, 8, 8)])
-def test_solve_matches_direct_solve(nelx, nely, nelz, precision, solver):
-    note("solve", nelx, nely, nelz, precision, solver)
+@pytest.mark.parametrize("nelx,nely,nelz", [(6, 4, 3), (12, 6, 5), (16, 8, 8), (23, 17, 9)])
+def test_solve_matches_direct_solve(nelx, nely, nelz, precision, solver, kind):
+    note("solve", nelx, nely, nelz, precision, solver, "operator_kind", kind)
     rng = np.random.default_rng(11)
     shape = (nely, nelx, nelz)
-    P, KE = make_plan(nelx, nely, nelz, 1.5)
+    P, KE = make_plan(nelx, nely, nelz, 1.5, options={"operator_kind": kind})
     rho = rng.uniform(0.01, 1.0, shape)
     F = oracle.edge_load_force_vector(nelx, nely, nelz)
     free, _ = oracle.support_dofs(nelx, nely, nelz)
     U_o = oracle.solve_displacement(rho, 3.0, KE, F, free, nelx, nely, nelz)
     rd, Fd = to_dev(elem_vec(rho)), to_dev(F)
     U = torch.zeros_like(Fd)
-    it, rr = P.solve(rd, 3.0, Fd, U, rtol=1e-13, maxit=20000, precision=precision,
-                     jacobi=(solver == "jacobi"))
+    it, rr = P.solve(rd, 3.0, Fd, U, rtol=1e-13 if precision == "fp64" else 1e-12,
+                     maxit=20000, precision=precision, jacobi=(solver == "jacobi"))
     note("  iters", it, "relres", rr)
     Ug = U.cpu().numpy()
     assert np.abs(Ug - U_o).max() / np.abs(U_o).max() < 1e-9
