@@ -114,6 +114,14 @@ int tk_filter_transpose2(tk_plan* plan, const double* g_c, const double* g_v,
  * of pytopo3d/core/optimizer.py:584-591. */
 int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const void* u, void* y,
                void* stream);
+/* One fine-grid Chebyshev smoothing step of the multigrid preconditioner,
+ * with the element scales and inverse diagonal of the last tk_solve in the
+ * same precision: d = c1 d + c2 D^-1 (b - K x); xo = x + d (identity rows on
+ * fixed DOFs).  b, x, xo, d: ndof doubles (TK_FP64) or floats (TK_MIXED_FP32).
+ * The smoother has no counterpart in the reference (its direct solve needs
+ * none); it is exported for timing and for callers building their own cycles. */
+int tk_smoother_step(tk_plan* plan, int prec, const void* b, const void* x, void* xo, void* d,
+                     double c1, double c2, void* stream);
 /* diag(K(rho)) (1 on fixed DOFs), ndof doubles. */
 int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void* stream);
 

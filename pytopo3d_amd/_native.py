@@ -62,6 +62,7 @@ _SIGS = {
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "tk_apply_K": (_I, [_P, _I, _P, _D, _P, _P, _P]),
     "tk_diag_K": (_I, [_P, _P, _D, _P, _P]),
+    "tk_smoother_step": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _P]),
     "tk_solve": (_I, [_P, _I, _P, _D, _P, _P, _D, _I, _I, ctypes.POINTER(_I),
                       ctypes.POINTER(_D), _P]),
     "tk_energy": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
@@ -202,6 +203,10 @@ class Plan:
     def apply_K(self, rho, penal, u, y, precision: str = "fp64"):
         _check(lib().tk_apply_K(self._h, PRECISIONS[precision], _ptr(rho), float(penal),
                                 _ptr(u), _ptr(y), _stream()))
+
+    def smoother_step(self, b, x, xo, d, c1, c2, precision="fp64"):
+        _check(lib().tk_smoother_step(self._h, PRECISIONS[precision], _ptr(b), _ptr(x), _ptr(xo),
+                                      _ptr(d), float(c1), float(c2), _stream()))
 
     def diag_K(self, rho, penal, diag):
         _check(lib().tk_diag_K(self._h, _ptr(rho), float(penal), _ptr(diag), _stream()))

@@ -16,6 +16,8 @@ void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
 int apply_fine(Plan& P, int prec, const void* u, void* y, hipStream_t s);
+int smooth_step_fine(Plan& P, int prec, const void* b, const void* x, void* xo, void* d,
+                     double c1, double c2, hipStream_t s);
 void launch_oc_init(Plan& P, double target, int max_steps, double tol, hipStream_t s);
 void launch_chain_weights(Plan& P, const double* gc, const double* gv, const double* drho,
                           double* wc, double* wv, hipStream_t s);
@@ -66,6 +68,7 @@ static int dmalloc(T** p, size_t n) {
 static void free_plan(Plan* P) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(P->KE64); f(P->KE32); f(P->sten_off); f(P->sten_w); f(P->Hs); f(P->eflags); f(P->fcols);
+  f(P->sec64_dev); f(P->sec32_dev);
   for (int l = 0; l < kMaxLevels; ++l) {
     Level& L = P->lv[l];
     f(L.fixed);
@@ -472,6 +475,24 @@ int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const v
     return 1;
   }
   return tk::apply_fine(P, prec, u, y, s);
+}
+
+int tk_smoother_step(tk_plan* plan, int prec, const void* b, const void* x, void* xo, void* d,
+                     double c1, double c2, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!b || !x || !xo || !d) {
+    set_error("b, x, xo and d are required");
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  if (P.last_penal < 0.0 || (prec == TK_MIXED_FP32 && P.last_prec == TK_FP64)) {
+    set_error("tk_smoother_step needs a previous tk_solve in this precision");
+    return 1;
+  }
+  return tk::smooth_step_fine(P, prec, b, x, xo, d, c1, c2, (hipStream_t)stream);
 }
 
 int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void* stream) {

@@ -119,6 +119,18 @@ __device__ __forceinline__ int block_id() {
 }
 __device__ __forceinline__ int block_count() { return (int)(gridDim.x * gridDim.y * gridDim.z); }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs of the chip, each with its own L2.  Remapping the linear block id so
+// that every XCD receives a contiguous range of logical tiles keeps the
+// overlapping halos of neighbouring tiles in the same L2 instead of fetching
+// them again from HBM.  The map is a bijection of [0, nb).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_logical(int bid, int nb) {
+  const int x = bid % kXcds, k = bid / kXcds;
+  const int q = nb / kXcds, r = nb % kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Two-phase grid reductions (the fast path): every block writes its partial
 // value(s) to partials[i * G + block] with no atomics, and a one-block
 // finalize kernel (tk_reduce.hip) combines them in a fixed order and applies

@@ -66,7 +66,10 @@ __global__ __launch_bounds__(256) void k_filter_tiled(
   }
   constexpr int TYH = FTY + 2 * H, TXH = FTX + 2 * H, TZH = FTZ + 2 * H;
   constexpr int zs = TYH * TXH;
-  const int y0 = blockIdx.x * FTY, x0 = blockIdx.y * FTX, z0 = blockIdx.z * FTZ;
+  // XCD-aware order: z, then y, then x neighbours of a tile share an L2
+  const int lg = xcd_logical(block_id(), block_count());
+  const int bz = lg % gridDim.z, rr = lg / gridDim.z;
+  const int y0 = (rr % gridDim.x) * FTY, x0 = (rr / gridDim.x) * FTX, z0 = bz * FTZ;
   const int exy = g.nelx * g.nely;
   constexpr int ntile = zs * TZH;
   for (int t = threadIdx.x; t < ntile; t += blockDim.x) {

@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c, const uint8
                                                       T* __restrict__ bc,
                                                       const int* __restrict__ done) {
   if (done && *done) return;
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (I >= c.nnode) return;
   int Iy, Ix, Iz;
   node_coords(c, I, Iy, Ix, Iz);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
                                                          T* __restrict__ xf,
                                                          const int* __restrict__ done) {
   if (done && *done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= f.nnode) return;
   int iy, ix, iz;
   node_coords(f, i, iy, ix, iz);
@@ -412,41 +412,55 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(Grid g, const double* __r
     __syncthreads();
   }
   for (int t = tid; t < np; t += nt) Lg[t] = A[t];
-  // inverse column j: L y = e_j, then L^T x = y (scratch = the output column)
-  for (int j = tid; j < n; j += nt) {
-    for (int i = 0; i < n; ++i) {
-      if (i < j) {
-        inv[i * n + j] = 0.0;
-        continue;
-      }
-      double acc = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; ++k) acc -= A[pk(i, k)] * inv[k * n + j];
-      inv[i * n + j] = acc / A[pk(i, i)];
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double acc = inv[i * n + j];
-      for (int k = i + 1; k < n; ++k) acc -= A[pk(k, i)] * inv[k * n + j];
-      inv[i * n + j] = acc / A[pk(i, i)];
-    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Column j of A^-1 = L^-T L^-1 e_j, one wave per column: forward and backward
+// substitution with the inner products spread over the 64 lanes.  A^-1 is
+// symmetric, so column j is stored as row j (coalesced by the coarse solve).
+__global__ __launch_bounds__(64) void k_coarse_inverse(int n, const double* __restrict__ Lg,
+                                                        double* __restrict__ inv64,
+                                                        float* __restrict__ inv32) {
+  __shared__ double y[kCoarseMaxDof];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  for (int i = lane; i < n; i += 64) y[i] = 0.0;
+  __syncthreads();
+  for (int i = j; i < n; ++i) {
+    double acc = 0.0;
+    for (int k = j + lane; k < i; k += 64) acc += Lg[pk(i, k)] * y[k];
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = ((i == j ? 1.0 : 0.0) - acc) / Lg[pk(i, i)];
+    __syncthreads();
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double acc = 0.0;
+    for (int k = i + 1 + lane; k < n; k += 64) acc += Lg[pk(k, i)] * y[k];
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = (y[i] - acc) / Lg[pk(i, i)];
+    __syncthreads();
+  }
+  for (int i = lane; i < n; i += 64) {
+    inv64[j * n + i] = y[i];
+    inv32[j * n + i] = (float)y[i];
   }
 }
 
-__global__ void k_inv_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
-}
-
+// x = A^-1 b on the coarsest grid: one wave per row.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const T* __restrict__ inv,
-                                                          const T* __restrict__ b,
-                                                          T* __restrict__ x,
-                                                          const int* __restrict__ done) {
+__global__ __launch_bounds__(64) void k_coarse_solve(int n, const T* __restrict__ inv,
+                                                      const T* __restrict__ b, T* __restrict__ x,
+                                                      const int* __restrict__ done) {
   if (done && *done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int i = blockIdx.x, lane = threadIdx.x;
   double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc += (double)inv[i * n + j] * (double)b[j];
-  x[i] = (T)acc;
+  for (int j = lane; j < n; j += 64) acc += (double)inv[i * n + j] * (double)b[j];
+  acc = wave_sum(acc);
+  if (lane == 0) x[i] = (T)acc;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -557,8 +571,8 @@ void launch_coarse_setup(Plan& P, hipStream_t s) {
   const size_t lds = sizeof(double) * (size_t)n * (n + 1) / 2;
   hipLaunchKernelGGL(k_coarse_setup, dim3(1), dim3(1024), lds, s, C.g, P.KE64, C.f64.s, C.fixed,
                      P.coarse_inv64, P.coarse_L);
-  hipLaunchKernelGGL(k_inv_to_f32, dim3(nblocks((long long)n * n)), dim3(kBlock), 0, s, n * n,
-                     P.coarse_inv64, P.coarse_inv32);
+  hipLaunchKernelGGL(k_coarse_inverse, dim3(n), dim3(64), 0, s, n, P.coarse_L, P.coarse_inv64,
+                     P.coarse_inv32);
 }
 
 template <typename T> static const T* inv_of(const Plan& P);
@@ -567,7 +581,7 @@ template <> const float* inv_of<float>(const Plan& P) { return P.coarse_inv32; }
 
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse_solve<T>, dim3(nblocks(P.ncoarse)), dim3(kBlock), 0, s, P.ncoarse,
+  hipLaunchKernelGGL(k_coarse_solve<T>, dim3(P.ncoarse), dim3(64), 0, s, P.ncoarse,
                      inv_of<T>(P), b, x, done_ptr(P));
 }
 

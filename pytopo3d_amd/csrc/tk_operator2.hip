@@ -78,7 +78,7 @@ __device__ __forceinline__ constexpr int t_of(int a) {
 
 // Element product: u[c][t] -> y[c][t] = s * (KE u)[c][t]
 template <typename T>
-__device__ __forceinline__ void element_apply(const SectorKE<T>& K, T s, T u[3][8]) {
+__device__ __forceinline__ void element_apply(const T* __restrict__ kb, T s, T u[3][8]) {
   wht_fwd(u[0]);
   wht_fwd(u[1]);
   wht_fwd(u[2]);
@@ -86,7 +86,7 @@ __device__ __forceinline__ void element_apply(const SectorKE<T>& K, T s, T u[3][
 #pragma unroll
   for (int chi = 0; chi < 8; ++chi) {
     const T a0 = u[0][chi ^ 1], a1 = u[1][chi ^ 2], a2 = u[2][chi ^ 4];
-    const T* b = K.b[chi];
+    const T* b = kb + 9 * chi;
     out[0][chi ^ 1] = s * (b[0] * a0 + b[1] * a1 + b[2] * a2);
     out[1][chi ^ 2] = s * (b[3] * a0 + b[4] * a1 + b[5] * a2);
     out[2][chi ^ 4] = s * (b[6] * a0 + b[7] * a1 + b[8] * a2);
@@ -107,25 +107,42 @@ __device__ __forceinline__ constexpr int slot_of_t(int t) {
   return (t & 3) - 1 + 3 * (t >> 2);
 }
 
+template <typename T>
+struct PlaneRegs {
+  T own[3];      // own node, unmasked
+  uint8_t fix;   // own node fixed bits (7 outside the domain)
+  T ext[3];      // extra halo node, masked
+};
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<T> K, const T* __restrict__ s,
+__global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<float> Kv,
+                                                  const T* __restrict__ Kg,
+                                                  const T* __restrict__ s,
                                                   const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ u, Epilogue<T> ep,
                                                   int nty, int zc, int with_dot,
                                                   const int* __restrict__ done, double* partials,
                                                   unsigned* counter, Scalars* sc) {
   if (done && *done) return;
-  __shared__ T un[3][kPl * kPl];          // one node plane of the tile (masked u)
+  __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked u)
   __shared__ T yo[6][3][kTile * kTile];   // exchanged corner contributions
+  __shared__ T kb[72];                    // sector blocks of KE
 
   const int tid = threadIdx.x;
+  // FP64 blocks are staged in LDS (144 SGPRs would spill); FP32 blocks stay in
+  // scalar registers (72 SGPRs), which measured faster than LDS broadcasts.
+  if (sizeof(T) == 8 && tid < 72) kb[tid] = Kg[tid];
   const int ly = tid & (kTile - 1), lx = tid >> 4;
-  const int Y0 = (blockIdx.x % nty) * kOut, X0 = (blockIdx.x / nty) * kOut;
-  const int Z0 = blockIdx.y * zc;
+  // XCD-aware order: z-chunks of a tile, then y, then x neighbours share an L2
+  const int lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int chunk = lg % gridDim.y, tile = lg / gridDim.y;
+  const int Y0 = (tile % nty) * kOut, X0 = (tile / nty) * kOut;
+  const int Z0 = chunk * zc;
   const int Z1 = min(Z0 + zc, g.nnz);
   const int ey = Y0 - 1 + ly, ex = X0 - 1 + lx;   // element == owned node (tile coords)
   const bool col_el = (unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx;
-  const bool owner = ly >= 1 && lx >= 1 && ey < g.nny && ex < g.nnx;
+  const bool col_node = (unsigned)ey < (unsigned)g.nny && (unsigned)ex < (unsigned)g.nnx;
+  const bool owner = ly >= 1 && lx >= 1 && col_node;
   const int nxy = g.nnx * g.nny;
   const int exy = g.nelx * g.nely;
 
@@ -133,86 +150,88 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<T> K, const T
   int xi = -1, xj = -1;
   if (tid < kPl) { xi = kTile; xj = tid; }
   else if (tid < kPl + kTile) { xi = tid - kPl; xj = kTile; }
+  const int xny = Y0 - 1 + xi, xnx = X0 - 1 + xj;
+  const bool ext_node = xi >= 0 && (unsigned)xny < (unsigned)g.nny && (unsigned)xnx < (unsigned)g.nnx;
 
-  auto load_plane = [&](int pz, T own_raw[3], uint8_t& own_fix) {
-    // own node
-    {
-      const int nyy = ey, nxx = ex;
-      T v[3] = {T(0), T(0), T(0)};
-      uint8_t f = 7;
-      if ((unsigned)pz < (unsigned)g.nnz && (unsigned)nyy < (unsigned)g.nny &&
-          (unsigned)nxx < (unsigned)g.nnx) {
-        const int n = nyy + nxx * g.nny + pz * nxy;
-        f = fixed[n];
-        v[0] = u[3 * n];
-        v[1] = u[3 * n + 1];
-        v[2] = u[3 * n + 2];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        own_raw[c] = v[c];
-        un[c][ly + kPl * lx] = ((f >> c) & 1) ? T(0) : v[c];
-      }
-      own_fix = f;
+  // global loads of node plane pz (registers only)
+  auto fetch = [&](int pz, PlaneRegs<T>& R) {
+    R.own[0] = R.own[1] = R.own[2] = T(0);
+    R.fix = 7;
+    if (col_node && (unsigned)pz < (unsigned)g.nnz) {
+      const int n = ey + ex * g.nny + pz * nxy;
+      R.fix = fixed[n];
+      R.own[0] = u[3 * n];
+      R.own[1] = u[3 * n + 1];
+      R.own[2] = u[3 * n + 2];
     }
-    if (xi >= 0) {
-      const int nyy = Y0 - 1 + xi, nxx = X0 - 1 + xj;
-      T v[3] = {T(0), T(0), T(0)};
-      if ((unsigned)pz < (unsigned)g.nnz && (unsigned)nyy < (unsigned)g.nny &&
-          (unsigned)nxx < (unsigned)g.nnx) {
-        const int n = nyy + nxx * g.nny + pz * nxy;
-        const uint8_t f = fixed[n];
-        v[0] = (f & 1) ? T(0) : u[3 * n];
-        v[1] = (f & 2) ? T(0) : u[3 * n + 1];
-        v[2] = (f & 4) ? T(0) : u[3 * n + 2];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) un[c][xi + kPl * xj] = v[c];
+    R.ext[0] = R.ext[1] = R.ext[2] = T(0);
+    if (ext_node && (unsigned)pz < (unsigned)g.nnz) {
+      const int n = xny + xnx * g.nny + pz * nxy;
+      const uint8_t f = fixed[n];
+      R.ext[0] = (f & 1) ? T(0) : u[3 * n];
+      R.ext[1] = (f & 2) ? T(0) : u[3 * n + 1];
+      R.ext[2] = (f & 4) ? T(0) : u[3 * n + 2];
     }
   };
+  auto store = [&](int buf, const PlaneRegs<T>& R) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[buf][c][ly + kPl * lx] = ((R.fix >> c) & 1) ? T(0) : R.own[c];
+    if (xi >= 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) un[buf][c][xi + kPl * xj] = R.ext[c];
+  };
 
-  // corner values of the element column: bottom (t = 0..3) and top (t = 4..7)
   T ue[3][8];
-  T raw_prev[3], raw_cur[3];
-  uint8_t fix_prev = 7, fix_cur = 7;
   T part[3] = {T(0), T(0), T(0)};   // partial sum of the node at the current top plane
   double dot = 0.0;
 
-  // plane Z0 - 1 -> bottom corners of the first element layer
-  load_plane(Z0 - 1, raw_prev, fix_prev);
+  PlaneRegs<T> cur, nxt;
+  fetch(Z0 - 1, cur);
+  fetch(Z0, nxt);
+  store(0, cur);
+  store(1, nxt);
+  T raw_prev[3] = {cur.own[0], cur.own[1], cur.own[2]};
+  uint8_t fix_prev = cur.fix;
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int oy = (t >> 1) & 1, ox = t & 1;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ue[c][t] = un[c][(ly + oy) + kPl * (lx + ox)];
+    for (int c = 0; c < 3; ++c) ue[c][t] = un[0][c][(ly + oy) + kPl * (lx + ox)];
   }
+  int buf = 1;   // buffer holding plane ez + 1
 
   for (int ez = Z0 - 1; ez < Z1; ++ez) {
-    __syncthreads();   // previous readers of un / yo are done
-    load_plane(ez + 1, raw_cur, fix_cur);
-    __syncthreads();
+    // issue the global loads of the next iteration and of this epilogue early
+    const bool more = ez + 1 < Z1;
+    PlaneRegs<T> pre;
+    if (more) fetch(ez + 2, pre);
+    const bool out_now = owner && ez >= Z0;
+    const int n_out = ey + ex * g.nny + ez * nxy;
+    T eb[3] = {T(0), T(0), T(0)}, ed[3] = {T(0), T(0), T(0)}, edi[3] = {T(0), T(0), T(0)};
+    if (out_now && MODE != kModeApply) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        eb[c] = ep.b[3 * n_out + c];
+        if (MODE == kModeChebFirst || MODE == kModeChebStep) edi[c] = ep.dinv[3 * n_out + c];
+        if (MODE == kModeChebStep) ed[c] = ep.d[3 * n_out + c];
+      }
+    }
+    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
+    const T se = el ? s[ey + ex * g.nely + ez * exy] : T(0);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int oy = (t >> 1) & 1, ox = t & 1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ue[c][t + 4] = un[c][(ly + oy) + kPl * (lx + ox)];
+      for (int c = 0; c < 3; ++c) ue[c][t + 4] = un[buf][c][(ly + oy) + kPl * (lx + ox)];
     }
     T w[3][8];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int t = 0; t < 8; ++t) w[c][t] = ue[c][t];
-    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
-    if (el) {
-      const T se = s[ey + ex * g.nely + ez * exy];
-      element_apply(K, se, w);
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) w[c][t] = T(0);
-    }
+    if constexpr (sizeof(T) == 8) element_apply<T>(kb, se, w);
+    else element_apply<T>(&Kv.b[0][0], se, w);   // se = 0 outside the domain gives zero contributions
 #pragma unroll
     for (int t = 1; t < 8; ++t) {
       if (t == 4) continue;
@@ -232,23 +251,22 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<T> K, const T
         top[c] = w[c][4] + yo[slot_of_t(6)][c][tym] + yo[slot_of_t(5)][c][txm] +
                  yo[slot_of_t(7)][c][tbm];
       }
-      if (ez >= Z0) {
-        const int n = ey + ex * g.nny + ez * nxy;
+      if (out_now) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const int i = 3 * n + c;
+          const int i = 3 * n_out + c;
           const T val = ((fix_prev >> c) & 1) ? raw_prev[c] : bot[c];
           if (MODE == kModeApply) {
             ep.y[i] = val;
             dot += (double)raw_prev[c] * (double)val;
           } else if (MODE == kModeResidual) {
-            ep.y[i] = ep.b[i] - val;
+            ep.y[i] = eb[c] - val;
           } else if (MODE == kModeChebFirst) {
-            const T dd = ep.c0 * ep.dinv[i] * (ep.b[i] - val);
+            const T dd = ep.c0 * edi[c] * (eb[c] - val);
             ep.d[i] = dd;
             ep.xo[i] = raw_prev[c] + dd;
           } else {
-            const T dd = ep.c1 * ep.d[i] + ep.c2 * ep.dinv[i] * (ep.b[i] - val);
+            const T dd = ep.c1 * ed[c] + ep.c2 * edi[c] * (eb[c] - val);
             ep.d[i] = dd;
             ep.xo[i] = raw_prev[c] + dd;
           }
@@ -261,9 +279,15 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<T> K, const T
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) ue[c][t] = ue[c][t + 4];
-      raw_prev[c] = raw_cur[c];
+      raw_prev[c] = nxt.own[c];
     }
-    fix_prev = fix_cur;
+    fix_prev = nxt.fix;
+    if (more) {
+      store(buf ^ 1, pre);
+      nxt = pre;
+    }
+    __syncthreads();
+    buf ^= 1;
   }
   if (MODE == kModeApply && with_dot) {
     double v[1] = {dot};
@@ -322,22 +346,17 @@ bool operator_setup(Plan& P) {
       P.sec64[chi][k] = sec[chi][k];
       P.sec32[chi][k] = (float)sec[chi][k];
     }
+  if (hipMalloc((void**)&P.sec64_dev, sizeof(double) * 72) != hipSuccess ||
+      hipMalloc((void**)&P.sec32_dev, sizeof(float) * 72) != hipSuccess ||
+      hipMemcpy(P.sec64_dev, P.sec64, sizeof(double) * 72, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P.sec32_dev, P.sec32, sizeof(float) * 72, hipMemcpyHostToDevice) != hipSuccess)
+    P.sector_ok = false;
   return P.sector_ok;
 }
 
-template <typename T> static SectorKE<T> sectors_of(const Plan& P);
-template <> SectorKE<double> sectors_of<double>(const Plan& P) {
-  SectorKE<double> k;
-  for (int i = 0; i < 8; ++i)
-    for (int j = 0; j < 9; ++j) k.b[i][j] = P.sec64[i][j];
-  return k;
-}
-template <> SectorKE<float> sectors_of<float>(const Plan& P) {
-  SectorKE<float> k;
-  for (int i = 0; i < 8; ++i)
-    for (int j = 0; j < 9; ++j) k.b[i][j] = P.sec32[i][j];
-  return k;
-}
+template <typename T> static const T* sectors_of(const Plan& P);
+template <> const double* sectors_of<double>(const Plan& P) { return P.sec64_dev; }
+template <> const float* sectors_of<float>(const Plan& P) { return P.sec32_dev; }
 
 static void el_grid(const Grid& g, int& nty, int& zc, dim3& grid) {
   nty = (g.nny + kOut - 1) / kOut;
@@ -366,7 +385,10 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
     zc = (L.g.nnz + nch - 1) / nch;
     grid.y = (L.g.nnz + zc - 1) / zc;
   }
-  hipLaunchKernelGGL((k_apply_el<T, MODE>), grid, dim3(256), 0, s, L.g, sectors_of<T>(P), sl,
+  SectorKE<float> kv;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 9; ++j) kv.b[i][j] = P.sec32[i][j];
+  hipLaunchKernelGGL((k_apply_el<T, MODE>), grid, dim3(256), 0, s, L.g, kv, sectors_of<T>(P), sl,
                      L.fixed, u, ep, nty, zc, with_dot, done, P.partials, P.counter, P.sc);
   if (MODE == kModeApply && with_dot) finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
 }

@@ -56,6 +56,8 @@ struct Plan {
   bool use_el = false;            // element kernel in use
   double sec64[8][9];             // sector blocks (W^T KE W / 64)
   float sec32[8][9];
+  double* sec64_dev = nullptr;    // device copies (staged into LDS by the element kernel)
+  float* sec32_dev = nullptr;
   // density filter
   double rmin = 0.0;
   int nsten = 0, R = 0;

@@ -430,4 +430,23 @@ int apply_fine(Plan& P, int prec, const void* u, void* y, hipStream_t s) {
   return 0;
 }
 
+// One fine-level Chebyshev smoothing step with the scales and inverse diagonal
+// of the last solve: d = c1 d + c2 D^-1 (b - K x); xo = x + d.
+int smooth_step_fine(Plan& P, int prec, const void* b, const void* x, void* xo, void* d,
+                     double c1, double c2, hipStream_t s) {
+  Level& L = P.lv[0];
+  if (!P.use_el) {
+    set_error("tk_smoother_step needs the element operator (mirror-symmetric KE)");
+    return 1;
+  }
+  if (prec == 0)
+    el_cheb<double>(P, L, L.f64.s, L.f64.dinv, (const double*)b, (const double*)x, (double*)xo,
+                    (double*)d, 0.0, c1, c2, 0, nullptr, s);
+  else
+    el_cheb<float>(P, L, L.f32.s, L.f32.dinv, (const float*)b, (const float*)x, (float*)xo,
+                   (float*)d, 0.0, c1, c2, 0, nullptr, s);
+  TK_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace tk
