@@ -12,8 +12,9 @@ stage).  A step is one full design iteration on the device: linear solve
 and the free-volume statistics, with the host reading the per-iteration
 scalars exactly as top3d does.
 
-The headline value is the mixed-precision path (FP32 operator and multigrid
-inside FP64 iterative refinement to a relative residual of 1e-8).  The same
+The headline value is the mixed-precision path (FP64 flexible conjugate
+gradients whose preconditioner is an FP32 multigrid V-cycle, to a relative
+residual of 1e-8).  The same
 steps are then timed on the FP64 path at the same tolerance, and the
 compliance / sensitivity error of the mixed solve against an FP64 solve of the
 same design is reported.  The roofline object is for the dominant kernel
@@ -137,22 +138,21 @@ def _time_kernels(problem, st, torch, args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / reps
 
-    # operator, FP32 and FP64: the last solve of each precision left its scales
+    # operator, FP32 and FP64, on the state tk_prepare builds for the design
     u32 = torch.randn(ndof, dtype=torch.float32, device=dev)
     y32 = torch.empty_like(u32)
     u64 = u32.double()
     y64 = torch.empty_like(u64)
-    U = torch.zeros(ndof, dtype=torch.float64, device=dev)
-    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="mixed")
+    plan.prepare(st.rho, 3.0, precision="mixed")
     t = timed(lambda: plan.apply_K(None, 3.0, u32, y32, precision="mixed"))
     out["operator_fp32"] = {"s": t, "bytes": 4 * (2 * ndof + nele), "flops": 1152 * nele,
                             "kernel": "k_apply_el<float, 0>"}
-    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="fp64")
+    plan.prepare(st.rho, 3.0, precision="fp64")
     t = timed(lambda: plan.apply_K(None, 3.0, u64, y64, precision="fp64"))
     out["operator_fp64"] = {"s": t, "bytes": 8 * (2 * ndof + nele), "flops": 1152 * nele,
                             "kernel": "k_apply_el<double, 0>"}
     # the dominant kernel of the mixed solve: the FP32 Chebyshev smoothing step
-    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="mixed")
+    plan.prepare(st.rho, 3.0, precision="mixed")
     b32 = torch.randn(ndof, dtype=torch.float32, device=dev)
     d32 = torch.zeros_like(b32)
     xo32 = torch.empty_like(b32)
@@ -167,26 +167,27 @@ def _time_kernels(problem, st, torch, args):
 
 
 def _kernel_probe(args):
-    """Fixed launches for counter collection: 5 x FP32 operator, 5 x FP64
-    operator, 5 x filter+projection and 5 x a 1 GiB device copy whose
-    FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    """Fixed launches for counter collection: 5 x FP32 operator, 5 x FP32
+    smoothing step, 5 x FP64 operator, 5 x filter+projection and 5 x a 1 GiB
+    device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters.  No solve
+    runs, so every launch of the operator kernels here is a full fine-grid
+    launch and their rocprofv3 average is the per-launch time."""
     problem, x0, target, DesignState, torch = _setup(args)
     plan = problem.plan
     st = DesignState(problem, x0)
     plan.filter_project(st.x, 1.0, 0.5, st.rf, st.rho, st.drho)
     ndof = plan.ndof
-    U = torch.zeros(ndof, dtype=torch.float64, device=problem.dev)
     u32 = torch.randn(ndof, dtype=torch.float32, device=problem.dev)
     y32 = torch.empty_like(u32)
     u64, y64 = u32.double(), torch.empty(ndof, dtype=torch.float64, device=problem.dev)
-    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="mixed")
+    plan.prepare(st.rho, 3.0, precision="mixed")
     torch.cuda.synchronize()
     for _ in range(5):
         plan.apply_K(None, 3.0, u32, y32, precision="mixed")
     b32, d32, xo32 = torch.randn_like(u32), torch.zeros_like(u32), torch.empty_like(u32)
     for _ in range(5):
         plan.smoother_step(b32, u32, xo32, d32, 0.5, 0.1, precision="mixed")
-    plan.solve(st.rho, 3.0, problem.F, U, rtol=1e-2, precision="fp64")
+    plan.prepare(st.rho, 3.0, precision="fp64")
     torch.cuda.synchronize()
     for _ in range(5):
         plan.apply_K(None, 3.0, u64, y64, precision="fp64")
@@ -289,8 +290,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"{args.nelx}x{args.nely}x{args.nelz} H8 cantilever (edge load), "
-                         "design iterations at beta=1, mixed FP32 multigrid-PCG in FP64 "
-                         f"iterative refinement to rtol {args.rtol:g}"),
+                         "design iterations at beta=1, FP64 flexible CG with an FP32 "
+                         f"multigrid correction, to rtol {args.rtol:g}"),
             "nelx": args.nelx, "nely": args.nely, "nelz": args.nelz,
             "volfrac": 0.3, "penal": 3.0, "rmin": 3.0, "beta": 1.0, "eta": 0.5, "move": 0.2,
             "solver_rtol": args.rtol, "levels": plan.levels(),

@@ -125,6 +125,15 @@ int tk_smoother_step(tk_plan* plan, int prec, const void* b, const void* x, void
 /* diag(K(rho)) (1 on fixed DOFs), ndof doubles. */
 int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void* stream);
 
+/* Build the solver state of a design without solving: element scales of every
+ * multigrid level, inverse diagonals and the coarsest-grid inverse, in the
+ * given precision (TK_MIXED_FP32 builds both).  A following tk_solve with
+ * TK_SOLVE_REUSE (same rho and penal), tk_apply_K with rho == NULL and
+ * tk_smoother_step use it.  This is the set-up half of the solve at
+ * pytopo3d/core/optimizer.py:584-594; it lets callers with several load cases
+ * per design build it once. */
+int tk_prepare(tk_plan* plan, int prec, const double* rho, double penal, void* stream);
+
 /* Solve K(rho) U = F on the free DOFs, U = 0 on fixed DOFs.
  * Replaces get_solver + the solve at pytopo3d/core/optimizer.py:591-594
  * (pytopo3d/utils/solver.py:18-73).  rtol: relative residual target; iters_out

@@ -63,6 +63,7 @@ _SIGS = {
     "tk_apply_K": (_I, [_P, _I, _P, _D, _P, _P, _P]),
     "tk_diag_K": (_I, [_P, _P, _D, _P, _P]),
     "tk_smoother_step": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _P]),
+    "tk_prepare": (_I, [_P, _I, _P, _D, _P]),
     "tk_solve": (_I, [_P, _I, _P, _D, _P, _P, _D, _I, _I, ctypes.POINTER(_I),
                       ctypes.POINTER(_D), _P]),
     "tk_energy": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
@@ -210,6 +211,11 @@ class Plan:
 
     def diag_K(self, rho, penal, diag):
         _check(lib().tk_diag_K(self._h, _ptr(rho), float(penal), _ptr(diag), _stream()))
+
+    def prepare(self, rho, penal, precision="fp64"):
+        """Build the multigrid state of a design; solve(..., reuse=True) uses it."""
+        _check(lib().tk_prepare(self._h, PRECISIONS[precision], _ptr(rho), float(penal),
+                                _stream()))
 
     def solve(self, rho, penal, F, U, rtol=1e-10, maxit=5000, precision="fp64",
               warm_start=False, jacobi=False, reuse=False):

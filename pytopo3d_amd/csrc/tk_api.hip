@@ -509,6 +509,23 @@ int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void
   return 0;
 }
 
+int tk_prepare(tk_plan* plan, int prec, const double* rho, double penal, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!rho) {
+    set_error("rho is required");
+    return 1;
+  }
+  if (!(std::isfinite(penal) && penal > 0)) {
+    set_error("penal must be positive, got " + std::to_string(penal));
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  return tk::setup_hierarchy(P, rho, penal, prec, (hipStream_t)stream);
+}
+
 int tk_solve(tk_plan* plan, int prec, const double* rho, double penal, const double* F,
              double* U, double rtol, int maxit, int flags, int* iters_out, double* relres_out,
              void* stream) {
@@ -529,8 +546,10 @@ int tk_solve(tk_plan* plan, int prec, const double* rho, double penal, const dou
     set_error("unknown precision");
     return 1;
   }
-  if ((flags & TK_SOLVE_REUSE) && P.last_penal != penal) {
-    set_error("TK_SOLVE_REUSE needs a previous solve on this plan with the same rho and penal");
+  if ((flags & TK_SOLVE_REUSE) &&
+      (P.last_penal != penal || (prec == TK_MIXED_FP32 && P.last_prec == TK_FP64))) {
+    set_error("TK_SOLVE_REUSE needs a previous solve or tk_prepare on this plan with the same rho, "
+              "penal and precision");
     return 1;
   }
   return tk::solve(P, prec, rho, penal, F, U, rtol, maxit, flags, iters_out, relres_out,

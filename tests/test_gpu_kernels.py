@@ -268,3 +268,27 @@ def test_reductions_are_bitwise_reproducible():
     assert torch.equal(outs[0][0], outs[1][0])
     assert outs[0][1] == outs[1][1]
     P.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_prepare_then_reuse_equals_solve(precision):
+    """tk_prepare + a TK_SOLVE_REUSE solve gives bitwise the result of a plain
+    solve, and a reuse after a set-up in another precision is refused."""
+    nelx, nely, nelz = 14, 8, 6
+    note("prepare/reuse", precision)
+    rng = np.random.default_rng(21)
+    P, _ = make_plan(nelx, nely, nelz, 1.5)
+    rho = to_dev(rng.uniform(0.01, 1, nelx * nely * nelz))
+    F = to_dev(oracle.edge_load_force_vector(nelx, nely, nelz))
+    Ua, Ub = torch.zeros_like(F), torch.zeros_like(F)
+    P.solve(rho, 3.0, F, Ua, rtol=1e-11, precision=precision)
+    P.prepare(rho, 3.0, precision=precision)
+    P.solve(rho, 3.0, F, Ub, rtol=1e-11, precision=precision, reuse=True)
+    assert torch.equal(Ua, Ub)
+    P.prepare(rho, 3.0, precision="fp64")
+    if precision == "mixed":
+        with pytest.raises(ValueError):
+            P.solve(rho, 3.0, F, Ub, rtol=1e-11, precision="mixed", reuse=True)
+    with pytest.raises(ValueError):
+        P.solve(rho, 2.0, F, Ub, rtol=1e-11, precision="fp64", reuse=True)
+    P.close()
