@@ -5,8 +5,8 @@
  * It replaces the per-iteration arithmetic of PyTopo3D's
  * pytopo3d.core.optimizer.top3d (pytopo3d/core/optimizer.py:578-696).  The
  * reference has no FFI of its own; every entry point below names the Python
- * function or call site it stands in for, and INTEGRATION.md shows the ctypes
- * binding.
+ * function or call site it stands in for; pytopo3d_amd/_native.py is the
+ * ctypes binding.
  *
  * Conventions
  *   - Array arguments are DEVICE pointers (contiguous, caller-owned), except

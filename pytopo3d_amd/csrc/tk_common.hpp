@@ -61,6 +61,7 @@ struct Scalars {
   double gray_cnt;     // count of 0.05 < rho < 0.95 over free elements
   double sum0, sum1;   // scratch sums
   double norm0, norm1; // scratch norms
+  double zs;           // scale of the FP32 preconditioner input (flexible CG)
 };
 
 // ---------------------------------------------------------------------------

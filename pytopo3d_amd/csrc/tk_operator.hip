@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(Grid g, const T* __restrict__ 
   if (with_dot) block_partials<1>(dot, partials);
 }
 
-// r = b - K x
+// r = b - K x on free DOFs, r = 0 on fixed DOFs (the restriction relies on it)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_residual(Grid g, const T* __restrict__ KE,
                                                       const T* __restrict__ s,
@@ -118,10 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(Grid g, const T* __restrict
   node_Ku(g, iy, ix, iz, ke, s, x, fixed, acc);
   const uint8_t f = fixed[n];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T yc = ((f >> c) & 1) ? x[3 * n + c] : acc[c];
-    r[3 * n + c] = b[3 * n + c] - yc;
-  }
+  for (int c = 0; c < 3; ++c) r[3 * n + c] = ((f >> c) & 1) ? T(0) : b[3 * n + c] - acc[c];
 }
 
 // First Chebyshev step: d = c0 Dinv (b - K x); xo = x + d  (x = 0: d = c0 Dinv b).
@@ -267,9 +264,13 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_fixed(Grid f, Grid c,
   fc[I] = v;
 }
 
-// b_c = R r_f = P^T r_f (trilinear weights), zero on coarse fixed DOFs.
+// b_c = R r_f = P^T r_f (trilinear weights), zero on coarse fixed DOFs.  The
+// fine residual is zero on fixed fine DOFs (the residual kernels write 0
+// there), so no fine mask is read.  Each coarse node gathers its 3x3x3 fine
+// neighbourhood with whole-node (3-component) loads; neighbouring threads
+// share fine nodes through L1.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c, const uint8_t* __restrict__ ff,
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
                                                       const uint8_t* __restrict__ fc,
                                                       const T* __restrict__ rf,
                                                       T* __restrict__ bc,
@@ -279,35 +280,40 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c, const uint8
   if (I >= c.nnode) return;
   int Iy, Ix, Iz;
   node_coords(c, I, Iy, Ix, Iz);
-  T acc[3] = {T(0), T(0), T(0)};
+  T a0 = T(0), a1 = T(0), a2 = T(0);
   const int nxy = f.nnx * f.nny;
+#pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
     const int iz = 2 * Iz + dz;
     if ((unsigned)iz >= (unsigned)f.nnz) continue;
     const T wz = dz ? T(0.5) : T(1);
+#pragma unroll
     for (int dx = -1; dx <= 1; ++dx) {
       const int ix = 2 * Ix + dx;
       if ((unsigned)ix >= (unsigned)f.nnx) continue;
       const T wzx = wz * (dx ? T(0.5) : T(1));
+      const T* __restrict__ row = rf + 3 * ((size_t)ix * f.nny + (size_t)iz * nxy);
 #pragma unroll
       for (int dy = -1; dy <= 1; ++dy) {
         const int iy = 2 * Iy + dy;
         if ((unsigned)iy >= (unsigned)f.nny) continue;
         const T wgt = wzx * (dy ? T(0.5) : T(1));
-        const int m = iy + ix * f.nny + iz * nxy;
-        const uint8_t fm = ff[m];
-        if (!(fm & 1)) acc[0] += wgt * rf[3 * m];
-        if (!(fm & 2)) acc[1] += wgt * rf[3 * m + 1];
-        if (!(fm & 4)) acc[2] += wgt * rf[3 * m + 2];
+        const T* __restrict__ q = row + 3 * iy;
+        a0 += wgt * q[0];
+        a1 += wgt * q[1];
+        a2 += wgt * q[2];
       }
     }
   }
   const uint8_t fI = fc[I];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) bc[3 * I + k] = ((fI >> k) & 1) ? T(0) : acc[k];
+  T* __restrict__ o = bc + 3 * (size_t)I;
+  o[0] = (fI & 1) ? T(0) : a0;
+  o[1] = (fI & 2) ? T(0) : a1;
+  o[2] = (fI & 4) ? T(0) : a2;
 }
 
-// x_f += P x_c on free fine DOFs.
+// x_f += P x_c on free fine DOFs.  Branch-free: an even fine index has one
+// coarse parent per axis, given weight 1 and a duplicate with weight 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
                                                          const uint8_t* __restrict__ ff,
@@ -322,27 +328,27 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
   const int y0 = iy >> 1, y1 = (iy + 1) >> 1;
   const int x0 = ix >> 1, x1 = (ix + 1) >> 1;
   const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
-  const T wy = (iy & 1) ? T(0.5) : T(1), wx = (ix & 1) ? T(0.5) : T(1),
-          wz = (iz & 1) ? T(0.5) : T(1);
+  const T wy[2] = {(iy & 1) ? T(0.5) : T(1), (iy & 1) ? T(0.5) : T(0)};
+  const T wx[2] = {(ix & 1) ? T(0.5) : T(1), (ix & 1) ? T(0.5) : T(0)};
+  const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
   const int cxy = c.nnx * c.nny;
-  T acc[3] = {T(0), T(0), T(0)};
+  T a0 = T(0), a1 = T(0), a2 = T(0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int Y = (k & 1) ? y1 : y0, X = ((k >> 1) & 1) ? x1 : x0, Z = (k >> 2) ? z1 : z0;
-    // duplicate corners (even index) contribute once
-    if ((k & 1) && y1 == y0) continue;
-    if (((k >> 1) & 1) && x1 == x0) continue;
-    if ((k >> 2) && z1 == z0) continue;
-    const T w = wy * wx * wz;
-    const int I = Y + X * c.nny + Z * cxy;
-    acc[0] += w * xc[3 * I];
-    acc[1] += w * xc[3 * I + 1];
-    acc[2] += w * xc[3 * I + 2];
+    const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+    const int I = (ky ? y1 : y0) + (kx ? x1 : x0) * c.nny + (kz ? z1 : z0) * cxy;
+    const T w = wy[ky] * wx[kx] * wz[kz];
+    const T* __restrict__ q = xc + 3 * (size_t)I;
+    a0 += w * q[0];
+    a1 += w * q[1];
+    a2 += w * q[2];
   }
   const uint8_t fi = ff[i];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if (!((fi >> k) & 1)) xf[3 * i + k] += acc[k];
+  T* __restrict__ o = xf + 3 * (size_t)i;
+  const T v0 = o[0], v1 = o[1], v2 = o[2];
+  o[0] = (fi & 1) ? v0 : v0 + a0;
+  o[1] = (fi & 2) ? v1 : v1 + a1;
+  o[2] = (fi & 4) ? v2 : v2 + a2;
 }
 
 // ------------------------------------------------------------------ coarsest grid
@@ -525,12 +531,12 @@ void launch_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* 
 
 template <typename T>
 void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x,
-                       T* xo, T* d, double c0, int x_zero, hipStream_t s) {
+                       T* xo, T* d, double c0, int x_zero, int write_d, hipStream_t s) {
   if (&L == &P.lv[0] && !x_zero) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
   }
   if (P.use_el && !x_zero) {
-    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, c0, 0.0, 0.0, 1, done_ptr(P), s);
+    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, c0, 0.0, 0.0, 1, write_d, done_ptr(P), s);
     return;
   }
   hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g,
@@ -539,12 +545,12 @@ void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, cons
 
 template <typename T>
 void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x,
-                      T* xo, T* d, double c1, double c2, hipStream_t s) {
+                      T* xo, T* d, double c1, double c2, int write_d, hipStream_t s) {
   if (&L == &P.lv[0]) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
   }
   if (P.use_el) {
-    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, 0.0, c1, c2, 0, done_ptr(P), s);
+    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, 0.0, c1, c2, 0, write_d, done_ptr(P), s);
     return;
   }
   hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g,
@@ -555,7 +561,7 @@ template <typename T>
 void launch_restrict(const Level& f, const Level& c, const T* rf, T* bc, hipStream_t s,
                      const int* done) {
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(c.g.nnode)), dim3(kBlock), 0, s, f.g, c.g,
-                     f.fixed, c.fixed, rf, bc, done);
+                     c.fixed, rf, bc, done);
 }
 
 template <typename T>
@@ -600,9 +606,9 @@ hipError_t coarse_setup_attr(size_t lds) {
   template void launch_residual<T>(Plan&, const Level&, const T*, const T*, const T*, T*,          \
                                    hipStream_t, const int*);                                       \
   template void launch_cheb_first<T>(Plan&, const Level&, const T*, const T*, const T*, const T*,  \
-                                     T*, T*, double, int, hipStream_t);                            \
+                                     T*, T*, double, int, int, hipStream_t);                       \
   template void launch_cheb_step<T>(Plan&, const Level&, const T*, const T*, const T*, const T*,   \
-                                    T*, T*, double, double, hipStream_t);                          \
+                                    T*, T*, double, double, int, hipStream_t);                     \
   template void launch_restrict<T>(const Level&, const Level&, const T*, T*, hipStream_t,          \
                                    const int*);                                                    \
   template void launch_prolong_add<T>(const Level&, const Level&, const T*, T*, hipStream_t,       \

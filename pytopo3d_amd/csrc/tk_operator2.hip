@@ -5,133 +5,187 @@
 // tests/test_stiffness.py:69-78).  In the Walsh-Hadamard basis of the 8 corner
 // values of each displacement component the 24x24 KE therefore splits into 8
 // independent 3x3 blocks, one per parity sector chi: the members of sector chi
-// are (component c, mode m = chi XOR e_c).  Per element:
-//     a = W^T u_e   (3 x 8-point butterflies, 72 add/sub)
-//     a'_chi = s_e B_chi a_chi   (8 x 3x3 blocks, 72 FMA + 24 mul)
-//     y_e = W a'    (72 add/sub)
-// with B = W^T KE W / 64 built and checked on the host.  This is about 240
-// operations per element instead of 1152 for the dense 24x24 product.
+// are (component c, mode m = chi XOR e_c).  Orthotropy adds zeros: the three
+// rigid translations (mode 0 of each component, in sectors 1, 2, 4) are null,
+// and in sectors 3, 5, 6 one component decouples, leaving 45 non-zero entries
+// (checked on the host, B = W^T KE W / 64).
+//
+// The Walsh-Hadamard transform factors into an in-plane (x, y) transform of
+// each 2x2 face and a z butterfly between the two faces of an element.  A
+// thread owns one element column and streams along z, so
+//   - the forward face transform of node plane k+1 (top face of element layer
+//     k) is reused as the bottom face of layer k+1, and
+//   - the top-face spectrum of layer k and the bottom-face spectrum of layer
+//     k+1 land on the same nodes, so they are summed before a single inverse
+//     face transform per node plane.
+// Per element layer: 24 (face forward) + 21 (z forward) + 45 FMA + 21 scale +
+// 21 (z inverse) + 12 (face sum) + 24 (face inverse) = 168 operations, against
+// 1152 for the dense 24x24 product.
 //
 // Scatter without atomics: a workgroup owns a 15 x 15 column of nodes in (y, x)
 // and streams along z.  Its 16 x 16 threads evaluate the 16 x 16 elements
 // touching those nodes (one halo row and column is re-evaluated by the
-// neighbouring tile), exchange the six off-diagonal corner contributions of
-// each element through LDS, and every node thread sums its 8 contributions in
-// a fixed order, so results are deterministic.  The top-plane partial sums of
-// one element layer stay in registers as the bottom-plane start of the next.
+// neighbouring tile), exchange the three off-diagonal corner contributions of
+// each node plane through LDS, and every node thread sums its 4 contributions
+// in a fixed order, so results are deterministic.
 #include "tk_plan.hpp"
 
 namespace tk {
 
+// Non-zero sector entries, 45 per KE, in this order (row-major within a block,
+// rows and columns = the member components listed):
+//   chi 0: comps {0,1,2} (9)  chi 1: {1,2} (4)  chi 2: {0,2} (4)
+//   chi 3: {0,1} (4) + comp 2 alone (1)  chi 4: {0,1} (4)
+//   chi 5: {0,2} (4) + comp 1 alone (1)  chi 6: {1,2} (4) + comp 0 alone (1)
+//   chi 7: {0,1,2} (9)
+constexpr int kSecN = 45;
+constexpr int kSecOff[8] = {0, 9, 13, 17, 22, 26, 31, 36};
+
 template <typename T>
-struct SectorKE {
-  T b[8][9];   // b[chi][3*c + c'], sector member c has mode chi ^ (1 << c)
+struct SparseKE {
+  T v[kSecN];
 };
 
 template <typename T>
 struct Epilogue {
   T* y;            // MODE 0/1 output
-  const T* b;      // MODE 1/2/3 right-hand side
-  T* xo;           // MODE 2/3 updated iterate
-  T* d;            // MODE 2/3 Chebyshev direction
-  const T* dinv;   // MODE 2/3 inverse diagonal
+  const T* b;      // MODE 1..4 right-hand side
+  T* xo;           // MODE 2..4 updated iterate
+  T* d;            // MODE 2..4 new Chebyshev direction (nullptr: not stored, last step)
+  const T* dp;     // MODE 3 previous Chebyshev direction
+  const T* dinv;   // MODE 2..4 inverse diagonal; MODE 2/3 nullptr: formed from kd and the
+                   // element scales (KE's diagonal is the same at every corner)
   T c0, c1, c2;
+  T kd[3];         // KE diagonal per component
 };
 
-enum { kModeApply = 0, kModeResidual = 1, kModeChebFirst = 2, kModeChebStep = 3 };
+// kModeChebZero fuses the first two Chebyshev steps from x = 0: the node
+// values are formed on the fly as x1 = c0 D^-1 b (= d1), then
+// d2 = c1 d1 + c2 D^-1 (b - K x1), x2 = x1 + d2.
+enum { kModeApply = 0, kModeResidual = 1, kModeChebFirst = 2, kModeChebStep = 3, kModeChebZero = 4 };
 
 constexpr int kTile = 16;          // threads per tile edge
 constexpr int kOut = kTile - 1;    // owned nodes per tile edge
 constexpr int kPl = kTile + 1;     // node-plane edge held in LDS
 
-// forward butterflies on 8 values in canonical corner order t = ox + 2 oy + 4 oz
+// In-plane transform of the 4 values of a face, index f = ox + 2 oy:
+// butterflies (lo, hi) -> (lo + hi, hi - lo) along x, then along y.
 template <typename T>
-__device__ __forceinline__ void wht_fwd(T v[8]) {
-#pragma unroll
-  for (int b = 1; b < 8; b <<= 1)
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-      if (!(t & b)) {
-        const T lo = v[t], hi = v[t | b];
-        v[t] = lo + hi;
-        v[t | b] = hi - lo;
-      }
+__device__ __forceinline__ void face_fwd(T v[4]) {
+  T a = v[0] + v[1], b = v[1] - v[0], c = v[2] + v[3], d = v[3] - v[2];
+  v[0] = a + c;
+  v[2] = c - a;
+  v[1] = b + d;
+  v[3] = d - b;
 }
 
+// Inverse butterflies (sm, df) -> (sm - df, sm + df) along x, then along y.
 template <typename T>
-__device__ __forceinline__ void wht_inv(T v[8]) {
-#pragma unroll
-  for (int b = 1; b < 8; b <<= 1)
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-      if (!(t & b)) {
-        const T sm = v[t], df = v[t | b];
-        v[t] = sm - df;
-        v[t | b] = sm + df;
-      }
+__device__ __forceinline__ void face_inv(T v[4]) {
+  T a = v[0] - v[1], b = v[0] + v[1], c = v[2] - v[3], d = v[2] + v[3];
+  v[0] = a - c;
+  v[2] = a + c;
+  v[1] = b - d;
+  v[3] = b + d;
 }
 
-// canonical index t of corner a, and back
-__device__ __forceinline__ constexpr int t_of(int a) {
-  return corner_x(a) + 2 * corner_y(a) + 4 * corner_z(a);
-}
-
-// Element product: u[c][t] -> y[c][t] = s * (KE u)[c][t]
+// Sector blocks: a[c][m] (spectrum, mode 0 unused) -> o[c][m] = s (B a)[c][m],
+// with o[c][0] = 0 implied (not written).
 template <typename T>
-__device__ __forceinline__ void element_apply(const T* __restrict__ kb, T s, T u[3][8]) {
-  wht_fwd(u[0]);
-  wht_fwd(u[1]);
-  wht_fwd(u[2]);
-  T out[3][8];
-#pragma unroll
-  for (int chi = 0; chi < 8; ++chi) {
-    const T a0 = u[0][chi ^ 1], a1 = u[1][chi ^ 2], a2 = u[2][chi ^ 4];
-    const T* b = kb + 9 * chi;
-    out[0][chi ^ 1] = s * (b[0] * a0 + b[1] * a1 + b[2] * a2);
-    out[1][chi ^ 2] = s * (b[3] * a0 + b[4] * a1 + b[5] * a2);
-    out[2][chi ^ 4] = s * (b[6] * a0 + b[7] * a1 + b[8] * a2);
+__device__ __forceinline__ void sector_apply(const T* __restrict__ k, T s, const T a[3][8],
+                                             T o[3][8]) {
+  // chi 0: members (0, 1), (1, 2), (2, 4)
+  {
+    const T* b = k + kSecOff[0];
+    const T x0 = a[0][1], x1 = a[1][2], x2 = a[2][4];
+    o[0][1] = s * (b[0] * x0 + b[1] * x1 + b[2] * x2);
+    o[1][2] = s * (b[3] * x0 + b[4] * x1 + b[5] * x2);
+    o[2][4] = s * (b[6] * x0 + b[7] * x1 + b[8] * x2);
   }
-  wht_inv(out[0]);
-  wht_inv(out[1]);
-  wht_inv(out[2]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int t = 0; t < 8; ++t) u[c][t] = out[c][t];
+  // chi 1: (1, 3), (2, 5)   [(0, 0) is the x translation]
+  {
+    const T* b = k + kSecOff[1];
+    const T x1 = a[1][3], x2 = a[2][5];
+    o[1][3] = s * (b[0] * x1 + b[1] * x2);
+    o[2][5] = s * (b[2] * x1 + b[3] * x2);
+  }
+  // chi 2: (0, 3), (2, 6)   [(1, 0) is the y translation]
+  {
+    const T* b = k + kSecOff[2];
+    const T x0 = a[0][3], x2 = a[2][6];
+    o[0][3] = s * (b[0] * x0 + b[1] * x2);
+    o[2][6] = s * (b[2] * x0 + b[3] * x2);
+  }
+  // chi 3: (0, 2), (1, 1) coupled; (2, 7) alone
+  {
+    const T* b = k + kSecOff[3];
+    const T x0 = a[0][2], x1 = a[1][1];
+    o[0][2] = s * (b[0] * x0 + b[1] * x1);
+    o[1][1] = s * (b[2] * x0 + b[3] * x1);
+    o[2][7] = s * (b[4] * a[2][7]);
+  }
+  // chi 4: (0, 5), (1, 6)   [(2, 0) is the z translation]
+  {
+    const T* b = k + kSecOff[4];
+    const T x0 = a[0][5], x1 = a[1][6];
+    o[0][5] = s * (b[0] * x0 + b[1] * x1);
+    o[1][6] = s * (b[2] * x0 + b[3] * x1);
+  }
+  // chi 5: (0, 4), (2, 1) coupled; (1, 7) alone
+  {
+    const T* b = k + kSecOff[5];
+    const T x0 = a[0][4], x2 = a[2][1];
+    o[0][4] = s * (b[0] * x0 + b[1] * x2);
+    o[2][1] = s * (b[2] * x0 + b[3] * x2);
+    o[1][7] = s * (b[4] * a[1][7]);
+  }
+  // chi 6: (1, 4), (2, 2) coupled; (0, 7) alone
+  {
+    const T* b = k + kSecOff[6];
+    const T x1 = a[1][4], x2 = a[2][2];
+    o[1][4] = s * (b[0] * x1 + b[1] * x2);
+    o[2][2] = s * (b[2] * x1 + b[3] * x2);
+    o[0][7] = s * (b[4] * a[0][7]);
+  }
+  // chi 7: (0, 6), (1, 5), (2, 3)
+  {
+    const T* b = k + kSecOff[7];
+    const T x0 = a[0][6], x1 = a[1][5], x2 = a[2][3];
+    o[0][6] = s * (b[0] * x0 + b[1] * x1 + b[2] * x2);
+    o[1][5] = s * (b[3] * x0 + b[4] * x1 + b[5] * x2);
+    o[2][3] = s * (b[6] * x0 + b[7] * x1 + b[8] * x2);
+  }
 }
 
-// Corner slots exchanged through LDS: t = 2 (x=0,y=1), 3 (x=1,y=1), 1 (x=1,y=0)
-// for the bottom plane, and the same +4 for the top plane.
-__device__ __forceinline__ constexpr int slot_of_t(int t) {
-  // t: 1 -> 0, 2 -> 1, 3 -> 2, 5 -> 3, 6 -> 4, 7 -> 5
-  return (t & 3) - 1 + 3 * (t >> 2);
-}
-
-template <typename T>
+template <typename T, int MODE>
 struct PlaneRegs {
-  T own[3];      // own node, unmasked
+  T own[3];      // own node operand, unmasked
   uint8_t fix;   // own node fixed bits (7 outside the domain)
-  T ext[3];      // extra halo node, masked
+  T ext[3];      // extra halo node operand, masked
+  T ob[3];       // kModeChebZero: own b and D^-1 (reused by the epilogue)
+  T odi[3];
 };
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<float> Kv,
+__global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
                                                   const T* __restrict__ Kg,
                                                   const T* __restrict__ s,
                                                   const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ u, Epilogue<T> ep,
                                                   int nty, int zc, int with_dot,
-                                                  const int* __restrict__ done, double* partials,
-                                                  unsigned* counter, Scalars* sc) {
+                                                  const int* __restrict__ done, double* partials) {
   if (done && *done) return;
-  __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked u)
-  __shared__ T yo[6][3][kTile * kTile];   // exchanged corner contributions
-  __shared__ T kb[72];                    // sector blocks of KE
+  __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked operand)
+  __shared__ T yo[3][3][kTile * kTile];   // exchanged corner contributions (f = 1, 2, 3)
+  __shared__ T kb[kSecN];                 // FP64 sector entries
+  __shared__ T ss[kTile * kTile];         // element scales of the layer (on-the-fly diagonal)
+  constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep;
+  const bool diag_fly = kCheb && ep.dinv == nullptr;
 
   const int tid = threadIdx.x;
-  // FP64 blocks are staged in LDS (144 SGPRs would spill); FP32 blocks stay in
-  // scalar registers (72 SGPRs), which measured faster than LDS broadcasts.
-  if (sizeof(T) == 8 && tid < 72) kb[tid] = Kg[tid];
+  // FP64 entries are staged in LDS (90 SGPRs would spill); FP32 entries stay
+  // in scalar registers.
+  if (sizeof(T) == 8 && tid < kSecN) kb[tid] = Kg[tid];
   const int ly = tid & (kTile - 1), lx = tid >> 4;
   // XCD-aware order: z-chunks of a tile, then y, then x neighbours share an L2
   const int lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -154,140 +208,195 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<float> Kv,
   const bool ext_node = xi >= 0 && (unsigned)xny < (unsigned)g.nny && (unsigned)xnx < (unsigned)g.nnx;
 
   // global loads of node plane pz (registers only)
-  auto fetch = [&](int pz, PlaneRegs<T>& R) {
-    R.own[0] = R.own[1] = R.own[2] = T(0);
+  auto fetch = [&](int pz, PlaneRegs<T, MODE>& R) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R.own[c] = R.ob[c] = R.odi[c] = T(0);
     R.fix = 7;
     if (col_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = ey + ex * g.nny + pz * nxy;
       R.fix = fixed[n];
-      R.own[0] = u[3 * n];
-      R.own[1] = u[3 * n + 1];
-      R.own[2] = u[3 * n + 2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (MODE == kModeChebZero) {
+          R.ob[c] = ep.b[3 * n + c];
+          R.odi[c] = ep.dinv[3 * n + c];
+          R.own[c] = ep.c0 * R.odi[c] * R.ob[c];
+        } else {
+          R.own[c] = u[3 * n + c];
+        }
+      }
     }
     R.ext[0] = R.ext[1] = R.ext[2] = T(0);
     if (ext_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = xny + xnx * g.nny + pz * nxy;
       const uint8_t f = fixed[n];
-      R.ext[0] = (f & 1) ? T(0) : u[3 * n];
-      R.ext[1] = (f & 2) ? T(0) : u[3 * n + 1];
-      R.ext[2] = (f & 4) ? T(0) : u[3 * n + 2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T v;
+        if constexpr (MODE == kModeChebZero) v = ep.c0 * ep.dinv[3 * n + c] * ep.b[3 * n + c];
+        else v = u[3 * n + c];
+        R.ext[c] = ((f >> c) & 1) ? T(0) : v;
+      }
     }
   };
-  auto store = [&](int buf, const PlaneRegs<T>& R) {
+  auto store = [&](int buf, const PlaneRegs<T, MODE>& R) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) un[buf][c][ly + kPl * lx] = ((R.fix >> c) & 1) ? T(0) : R.own[c];
     if (xi >= 0)
 #pragma unroll
       for (int c = 0; c < 3; ++c) un[buf][c][xi + kPl * xj] = R.ext[c];
   };
+  // face values of node plane buf at this thread's element column, then the
+  // in-plane transform
+  auto face_spectrum = [&](int buf, T F[3][4]) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int ox = f & 1, oy = f >> 1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c][f] = un[buf][c][(ly + oy) + kPl * (lx + ox)];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) face_fwd(F[c]);
+  };
 
-  T ue[3][8];
-  T part[3] = {T(0), T(0), T(0)};   // partial sum of the node at the current top plane
   double dot = 0.0;
-
-  PlaneRegs<T> cur, nxt;
-  fetch(Z0 - 1, cur);
-  fetch(Z0, nxt);
-  store(0, cur);
-  store(1, nxt);
-  T raw_prev[3] = {cur.own[0], cur.own[1], cur.own[2]};
-  uint8_t fix_prev = cur.fix;
+  PlaneRegs<T, MODE> RA, RB;
+  fetch(Z0 - 1, RB);
+  fetch(Z0, RA);
+  store(0, RB);
+  store(1, RA);
+  // own node of the plane being finished (the bottom plane of the layer)
+  T raw_prev[3] = {RB.own[0], RB.own[1], RB.own[2]};
+  T pb_prev[3] = {RB.ob[0], RB.ob[1], RB.ob[2]};
+  T pdi_prev[3] = {RB.odi[0], RB.odi[1], RB.odi[2]};
+  uint8_t fix_prev = RB.fix;
   __syncthreads();
+  T F0[3][4], F1[3][4];   // face spectra of two consecutive node planes
+  T Gp[3][4];             // top-face output spectrum of the previous element layer
+  T s_prev = T(0);        // on-the-fly diagonal: scale sum of the previous layer
+  face_spectrum(0, F0);
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int oy = (t >> 1) & 1, ox = t & 1;
+  for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ue[c][t] = un[0][c][(ly + oy) + kPl * (lx + ox)];
-  }
-  int buf = 1;   // buffer holding plane ez + 1
+    for (int f = 0; f < 4; ++f) Gp[c][f] = T(0);
 
-  for (int ez = Z0 - 1; ez < Z1; ++ez) {
-    // issue the global loads of the next iteration and of this epilogue early
+  // Element layer ez.  Fb: spectrum of plane ez (bottom), Ft: filled with that
+  // of plane ez + 1 (LDS buffer `buf`); nxt holds plane ez + 1 (registers),
+  // pre receives plane ez + 2.  Consecutive layers swap the roles of the
+  // buffer pairs, so the z-loop is unrolled by two and nothing is copied.
+  auto layer = [&](int ez, int buf, T (&Fb)[3][4], T (&Ft)[3][4], const PlaneRegs<T, MODE>& nxt,
+                   PlaneRegs<T, MODE>& pre) {
+    // issue the global loads of the next layer and of this epilogue early
     const bool more = ez + 1 < Z1;
-    PlaneRegs<T> pre;
     if (more) fetch(ez + 2, pre);
     const bool out_now = owner && ez >= Z0;
     const int n_out = ey + ex * g.nny + ez * nxy;
     T eb[3] = {T(0), T(0), T(0)}, ed[3] = {T(0), T(0), T(0)}, edi[3] = {T(0), T(0), T(0)};
-    if (out_now && MODE != kModeApply) {
+    if constexpr (MODE == kModeChebZero) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        eb[c] = ep.b[3 * n_out + c];
-        if (MODE == kModeChebFirst || MODE == kModeChebStep) edi[c] = ep.dinv[3 * n_out + c];
-        if (MODE == kModeChebStep) ed[c] = ep.d[3 * n_out + c];
+        eb[c] = pb_prev[c];
+        edi[c] = pdi_prev[c];
       }
-    }
-    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
-    const T se = el ? s[ey + ex * g.nely + ez * exy] : T(0);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int oy = (t >> 1) & 1, ox = t & 1;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ue[c][t + 4] = un[buf][c][(ly + oy) + kPl * (lx + ox)];
-    }
-    T w[3][8];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) w[c][t] = ue[c][t];
-    if constexpr (sizeof(T) == 8) element_apply<T>(kb, se, w);
-    else element_apply<T>(&Kv.b[0][0], se, w);   // se = 0 outside the domain gives zero contributions
-#pragma unroll
-    for (int t = 1; t < 8; ++t) {
-      if (t == 4) continue;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) yo[slot_of_t(t)][c][tid] = w[c][t];
-    }
-    __syncthreads();
-    // node (ly, lx) at the bottom plane ez: own corner t=0, (ly-1,lx) t=2,
-    // (ly,lx-1) t=1, (ly-1,lx-1) t=3; top plane: the same + 4.
-    if (owner) {
-      const int tym = tid - 1, txm = tid - kTile, tbm = tid - kTile - 1;
-      T bot[3], top[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        bot[c] = part[c] + w[c][0] + yo[slot_of_t(2)][c][tym] + yo[slot_of_t(1)][c][txm] +
-                 yo[slot_of_t(3)][c][tbm];
-        top[c] = w[c][4] + yo[slot_of_t(6)][c][tym] + yo[slot_of_t(5)][c][txm] +
-                 yo[slot_of_t(7)][c][tbm];
-      }
+    } else if (MODE != kModeApply) {
       if (out_now) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const int i = 3 * n_out + c;
-          const T val = ((fix_prev >> c) & 1) ? raw_prev[c] : bot[c];
-          if (MODE == kModeApply) {
-            ep.y[i] = val;
-            dot += (double)raw_prev[c] * (double)val;
-          } else if (MODE == kModeResidual) {
-            ep.y[i] = eb[c] - val;
-          } else if (MODE == kModeChebFirst) {
-            const T dd = ep.c0 * edi[c] * (eb[c] - val);
-            ep.d[i] = dd;
-            ep.xo[i] = raw_prev[c] + dd;
-          } else {
-            const T dd = ep.c1 * ed[c] + ep.c2 * edi[c] * (eb[c] - val);
-            ep.d[i] = dd;
-            ep.xo[i] = raw_prev[c] + dd;
-          }
+          eb[c] = ep.b[3 * n_out + c];
+          if (MODE != kModeResidual && !diag_fly) edi[c] = ep.dinv[3 * n_out + c];
+          if (MODE == kModeChebStep) ed[c] = ep.dp[3 * n_out + c];
         }
       }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) part[c] = top[c];
     }
+    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
+    const T se = el ? s[ey + ex * g.nely + ez * exy] : T(0);   // 0 gives zero contributions
+
+    // forward transform, sector blocks, inverse z butterfly
+    face_spectrum(buf, Ft);
+    T a[3][8], o[3][8];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) ue[c][t] = ue[c][t + 4];
+      for (int f = 1; f < 4; ++f) a[c][f] = Fb[c][f] + Ft[c][f];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
+    }
+    if constexpr (sizeof(T) == 8) sector_apply<T>(kb, se, a, o);
+    else sector_apply<T>(Kv.v, se, a, o);
+    // bottom plane: sum with the previous layer's top, inverse face transform
+    T nb[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      nb[c][0] = Gp[c][0] - o[c][4];
+      Gp[c][0] = o[c][4];
+#pragma unroll
+      for (int f = 1; f < 4; ++f) {
+        nb[c][f] = Gp[c][f] + (o[c][f] - o[c][f + 4]);
+        Gp[c][f] = o[c][f] + o[c][f + 4];
+      }
+      face_inv(nb[c]);
+    }
+#pragma unroll
+    for (int f = 1; f < 4; ++f)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) yo[f - 1][c][tid] = nb[c][f];
+    if (diag_fly) ss[tid] = se;
+    __syncthreads();
+    if (diag_fly && owner) {
+      // diag(K) at node (ly, lx) of plane ez: kd[c] times the scales of the 8
+      // elements around it (4 in layer ez - 1, carried in s_prev, 4 in layer ez)
+      const T scur = se + ss[tid - 1] + ss[tid - kTile] + ss[tid - kTile - 1];
+      const T dsum = s_prev + scur;
+      s_prev = scur;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) edi[c] = ((fix_prev >> c) & 1) ? T(1) : T(1) / (ep.kd[c] * dsum);
+    }
+    // node (ly, lx) of plane ez: own corner f = 0, element (ly-1, lx) its f = 2
+    // (oy = 1), element (ly, lx-1) its f = 1, element (ly-1, lx-1) its f = 3
+    if (out_now) {
+      const int tym = tid - 1, txm = tid - kTile, tbm = tid - kTile - 1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ku = nb[c][0] + yo[1][c][tym] + yo[0][c][txm] + yo[2][c][tbm];
+        const int i = 3 * n_out + c;
+        const bool fx = (fix_prev >> c) & 1;
+        const T val = fx ? raw_prev[c] : ku;
+        if (MODE == kModeApply) {
+          ep.y[i] = val;
+          dot += (double)raw_prev[c] * (double)val;
+        } else if (MODE == kModeResidual) {
+          // zero on fixed DOFs: the restriction reads no fine mask
+          ep.y[i] = fx ? T(0) : eb[c] - val;
+        } else if (MODE == kModeChebFirst) {
+          const T dd = ep.c0 * edi[c] * (eb[c] - val);
+          if (ep.d) ep.d[i] = dd;
+          ep.xo[i] = raw_prev[c] + dd;
+        } else if (MODE == kModeChebStep) {
+          const T dd = ep.c1 * ed[c] + ep.c2 * edi[c] * (eb[c] - val);
+          if (ep.d) ep.d[i] = dd;
+          ep.xo[i] = raw_prev[c] + dd;
+        } else {   // kModeChebZero: d1 = x1 = raw_prev
+          const T dd = ep.c1 * raw_prev[c] + ep.c2 * edi[c] * (eb[c] - val);
+          if (ep.d) ep.d[i] = dd;
+          ep.xo[i] = raw_prev[c] + dd;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
       raw_prev[c] = nxt.own[c];
+      pb_prev[c] = nxt.ob[c];
+      pdi_prev[c] = nxt.odi[c];
     }
     fix_prev = nxt.fix;
-    if (more) {
-      store(buf ^ 1, pre);
-      nxt = pre;
-    }
+    if (more) store(buf ^ 1, pre);
     __syncthreads();
-    buf ^= 1;
+  };
+
+  for (int ez = Z0 - 1; ez < Z1; ez += 2) {
+    layer(ez, 1, F0, F1, RA, RB);
+    if (ez + 1 >= Z1) break;
+    layer(ez + 1, 0, F1, F0, RB, RA);
   }
   if (MODE == kModeApply && with_dot) {
     double v[1] = {dot};
@@ -297,8 +406,9 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SectorKE<float> Kv,
 
 // ---------------------------------------------------------------------------
 // Host: sector decomposition of KE.  Returns false when KE is not
-// reflection-symmetric (then the dense node kernel is used).
-static bool build_sectors(const double* KE, double out[8][9]) {
+// reflection-symmetric with the orthotropic zero pattern (then the dense node
+// kernel is used).
+static bool build_sectors(const double* KE, double out[kSecN]) {
   // W[(a,c),(m,c)] = prod_d sigma_{a,d}^{m_d}, sigma = -1 (offset 0) / +1 (offset 1)
   auto psi = [](int a, int m) {
     const int o[3] = {corner_x(a), corner_y(a), corner_z(a)};
@@ -321,35 +431,49 @@ static bool build_sectors(const double* KE, double out[8][9]) {
           B[3 * m + c][3 * m2 + c2] = acc / 64.0;
           bmax = fmax(bmax, fabs(acc / 64.0));
         }
-  // sector membership: (c, m) belongs to chi = m ^ (1 << c)
+  // the 45 kept entries, as (chi, c, c') in the kernel's order
+  static const int keep[kSecN][3] = {
+      {0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 0}, {0, 1, 1}, {0, 1, 2}, {0, 2, 0}, {0, 2, 1},
+      {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 1}, {1, 2, 2}, {2, 0, 0}, {2, 0, 2}, {2, 2, 0},
+      {2, 2, 2}, {3, 0, 0}, {3, 0, 1}, {3, 1, 0}, {3, 1, 1}, {3, 2, 2}, {4, 0, 0}, {4, 0, 1},
+      {4, 1, 0}, {4, 1, 1}, {5, 0, 0}, {5, 0, 2}, {5, 2, 0}, {5, 2, 2}, {5, 1, 1}, {6, 1, 1},
+      {6, 1, 2}, {6, 2, 1}, {6, 2, 2}, {6, 0, 0}, {7, 0, 0}, {7, 0, 1}, {7, 0, 2}, {7, 1, 0},
+      {7, 1, 1}, {7, 1, 2}, {7, 2, 0}, {7, 2, 1}, {7, 2, 2}};
+  bool kept[24][24] = {};
+  for (int k = 0; k < kSecN; ++k) {
+    const int chi = keep[k][0], c = keep[k][1], c2 = keep[k][2];
+    const int m = chi ^ (1 << c), m2 = chi ^ (1 << c2);
+    out[k] = B[3 * m + c][3 * m2 + c2];
+    kept[3 * m + c][3 * m2 + c2] = true;
+  }
   double off = 0.0;
   for (int i = 0; i < 24; ++i)
-    for (int j = 0; j < 24; ++j) {
-      const int ci = i % 3, mi = i / 3, cj = j % 3, mj = j / 3;
-      if ((mi ^ (1 << ci)) != (mj ^ (1 << cj))) off = fmax(off, fabs(B[i][j]));
-    }
-  if (off > 1e-12 * bmax) return false;
-  for (int chi = 0; chi < 8; ++chi)
-    for (int c = 0; c < 3; ++c)
-      for (int c2 = 0; c2 < 3; ++c2) {
-        const int m = chi ^ (1 << c), m2 = chi ^ (1 << c2);
-        out[chi][3 * c + c2] = B[3 * m + c][3 * m2 + c2];
-      }
-  return true;
+    for (int j = 0; j < 24; ++j)
+      if (!kept[i][j]) off = fmax(off, fabs(B[i][j]));
+  return off <= 1e-12 * bmax;
 }
 
 bool operator_setup(Plan& P) {
-  double sec[8][9];
+  double sec[kSecN];
   P.sector_ok = build_sectors(P.KE, sec);
-  for (int chi = 0; chi < 8; ++chi)
-    for (int k = 0; k < 9; ++k) {
-      P.sec64[chi][k] = sec[chi][k];
-      P.sec32[chi][k] = (float)sec[chi][k];
+  // KE diagonal: the same at every corner for each component?
+  double dmax = 0.0, dev = 0.0;
+  for (int c = 0; c < 3; ++c) P.kd[c] = P.KE[c * 24 + c];
+  for (int a = 0; a < 8; ++a)
+    for (int c = 0; c < 3; ++c) {
+      const double v = P.KE[(3 * a + c) * 24 + 3 * a + c];
+      dmax = fmax(dmax, fabs(v));
+      dev = fmax(dev, fabs(v - P.kd[c]));
     }
-  if (hipMalloc((void**)&P.sec64_dev, sizeof(double) * 72) != hipSuccess ||
-      hipMalloc((void**)&P.sec32_dev, sizeof(float) * 72) != hipSuccess ||
-      hipMemcpy(P.sec64_dev, P.sec64, sizeof(double) * 72, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(P.sec32_dev, P.sec32, sizeof(float) * 72, hipMemcpyHostToDevice) != hipSuccess)
+  P.kd_ok = dev <= 1e-13 * dmax && dmax > 0.0;
+  for (int k = 0; k < kSecN; ++k) {
+    P.sec64[k] = sec[k];
+    P.sec32[k] = (float)sec[k];
+  }
+  if (hipMalloc((void**)&P.sec64_dev, sizeof(double) * kSecN) != hipSuccess ||
+      hipMalloc((void**)&P.sec32_dev, sizeof(float) * kSecN) != hipSuccess ||
+      hipMemcpy(P.sec64_dev, P.sec64, sizeof(double) * kSecN, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P.sec32_dev, P.sec32, sizeof(float) * kSecN, hipMemcpyHostToDevice) != hipSuccess)
     P.sector_ok = false;
   return P.sector_ok;
 }
@@ -361,13 +485,15 @@ template <> const float* sectors_of<float>(const Plan& P) { return P.sec32_dev; 
 static void el_grid(const Grid& g, int& nty, int& zc, dim3& grid) {
   nty = (g.nny + kOut - 1) / kOut;
   const int ntx = (g.nnx + kOut - 1) / kOut;
-  // z chunks: enough workgroups to fill the chip, few re-evaluated halo layers
+  // z chunks: about 2048 workgroups.  On the fine grid that streams ~10 node
+  // planes per workgroup (one re-evaluated element layer per chunk); on the
+  // coarse grids, whose launches are latency-bound, chunks shrink to 1-2
+  // planes so that the serial z-loop stays short.
   const int tiles = nty * ntx;
   int nch = (2048 + tiles - 1) / tiles;
   if (nch > g.nnz) nch = g.nnz;
   if (nch < 1) nch = 1;
-  int zc0 = (g.nnz + nch - 1) / nch;
-  if (zc0 < 8 && g.nnz >= 8) zc0 = 8;
+  const int zc0 = (g.nnz + nch - 1) / nch;
   nch = (g.nnz + zc0 - 1) / zc0;
   zc = (g.nnz + nch - 1) / nch;
   grid = dim3(tiles, nch);
@@ -385,11 +511,10 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
     zc = (L.g.nnz + nch - 1) / nch;
     grid.y = (L.g.nnz + zc - 1) / zc;
   }
-  SectorKE<float> kv;
-  for (int i = 0; i < 8; ++i)
-    for (int j = 0; j < 9; ++j) kv.b[i][j] = P.sec32[i][j];
+  SparseKE<float> kv;
+  for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
   hipLaunchKernelGGL((k_apply_el<T, MODE>), grid, dim3(256), 0, s, L.g, kv, sectors_of<T>(P), sl,
-                     L.fixed, u, ep, nty, zc, with_dot, done, P.partials, P.counter, P.sc);
+                     L.fixed, u, ep, nty, zc, with_dot, done, P.partials);
   if (MODE == kModeApply && with_dot) finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
 }
 
@@ -412,17 +537,35 @@ void el_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T
 
 template <typename T>
 void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x, T* xo,
-             T* d, double c0, double c1, double c2, int first, const int* done, hipStream_t s) {
+             T* d, double c0, double c1, double c2, int first, int write_d, const int* done,
+             hipStream_t s) {
   Epilogue<T> ep{};
   ep.b = b;
   ep.xo = xo;
-  ep.d = d;
-  ep.dinv = dinv;
+  ep.d = write_d ? d : nullptr;
+  ep.dp = d;
+  ep.dinv = P.kd_ok ? nullptr : dinv;
+  for (int c = 0; c < 3; ++c) ep.kd[c] = (T)P.kd[c];
   ep.c0 = (T)c0;
   ep.c1 = (T)c1;
   ep.c2 = (T)c2;
   if (first) launch_el<T, kModeChebFirst>(P, L, sl, x, ep, 0, done, s);
   else launch_el<T, kModeChebStep>(P, L, sl, x, ep, 0, done, s);
+}
+
+// First two Chebyshev steps from x = 0 in one sweep (see kModeChebZero).
+template <typename T>
+void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, T* xo, T* d,
+                   double c0, double c1, double c2, int write_d, const int* done, hipStream_t s) {
+  Epilogue<T> ep{};
+  ep.b = b;
+  ep.xo = xo;
+  ep.d = write_d ? d : nullptr;
+  ep.dinv = dinv;
+  ep.c0 = (T)c0;
+  ep.c1 = (T)c1;
+  ep.c2 = (T)c2;
+  launch_el<T, kModeChebZero>(P, L, sl, nullptr, ep, 0, done, s);
 }
 
 #define TK_INST2(T)                                                                              \
@@ -431,7 +574,9 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   template void el_residual<T>(Plan&, const Level&, const T*, const T*, const T*, T*,           \
                                const int*, hipStream_t);                                        \
   template void el_cheb<T>(Plan&, const Level&, const T*, const T*, const T*, const T*, T*, T*, \
-                           double, double, double, int, const int*, hipStream_t);
+                           double, double, double, int, int, const int*, hipStream_t);  \
+  template void el_cheb_zero2<T>(Plan&, const Level&, const T*, const T*, const T*, T*, T*,     \
+                                 double, double, double, int, const int*, hipStream_t);
 TK_INST2(double)
 TK_INST2(float)
 #undef TK_INST2

@@ -54,8 +54,10 @@ struct Plan {
   double lam_bound = 0.0;         // lambda_max(diag(KE)^-1 KE): bound of lambda_max(D^-1 K)
   bool sector_ok = false;         // KE block-diagonalises in the Walsh-Hadamard basis
   bool use_el = false;            // element kernel in use
-  double sec64[8][9];             // sector blocks (W^T KE W / 64)
-  float sec32[8][9];
+  double sec64[45];               // non-zero sector entries of W^T KE W / 64
+  float sec32[45];
+  double kd[3] = {0, 0, 0};       // KE diagonal per component
+  bool kd_ok = false;             // ... equal at all 8 corners (diagonal formed on the fly)
   double* sec64_dev = nullptr;    // device copies (staged into LDS by the element kernel)
   float* sec32_dev = nullptr;
   // density filter
@@ -138,12 +140,14 @@ void launch_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, double
 template <typename T>
 void launch_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T* r,
                      hipStream_t s, const int* done);
+// write_d = 0 (the last step of a smoothing pass) lets the element kernel skip
+// storing the direction d, which nothing reads afterwards.
 template <typename T>
 void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
-                       const T* x, T* xo, T* d, double c0, int x_zero, hipStream_t s);
+                       const T* x, T* xo, T* d, double c0, int x_zero, int write_d, hipStream_t s);
 template <typename T>
 void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
-                      const T* x, T* xo, T* d, double c1, double c2, hipStream_t s);
+                      const T* x, T* xo, T* d, double c1, double c2, int write_d, hipStream_t s);
 template <typename T>
 void launch_restrict(const Level& f, const Level& c, const T* rf, T* bc, hipStream_t s,
                      const int* done);
@@ -168,7 +172,11 @@ void el_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T
                  const int* done, hipStream_t s);
 template <typename T>
 void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x, T* xo,
-             T* d, double c0, double c1, double c2, int first, const int* done, hipStream_t s);
+             T* d, double c0, double c1, double c2, int first, int write_d, const int* done,
+             hipStream_t s);
+template <typename T>
+void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, T* xo, T* d,
+                   double c0, double c1, double c2, int write_d, const int* done, hipStream_t s);
 void launch_coarse_setup(Plan& P, hipStream_t s);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       sc->bb = s[0];
       sc->rr = s[1];
       sc->rr0 = s[1];
+      sc->zs = sqrt(s[1]);
       sc->tol2 = a0;
       sc->maxit = i0;
       sc->iter = 0;
@@ -56,6 +57,7 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       if (!(s[1] == s[1])) { sc->done = 1; sc->stalled = 1; }
       break;
     case kFinUpdate:
+      sc->zs = sqrt(sc->rr);   // the norm the fused update scaled the FP32 residual by
       sc->rr = s[0];
       sc->iter += 1;
       if (s[0] <= sc->tol2 * sc->bb) sc->done = 1;

@@ -1,6 +1,6 @@
 // Linear solve K U = F on the free DOFs: preconditioned conjugate gradients
-// with a geometric-multigrid V-cycle (or Jacobi), in FP64, or as FP64
-// iterative refinement around an FP32 inner solve.
+// with a geometric-multigrid V-cycle (or Jacobi), in FP64, or as FP64 flexible
+// conjugate gradients around an FP32 multigrid (or Jacobi) correction.
 //
 // Replaces the solver factory of pytopo3d/utils/solver.py:18-73 (PyPardiso /
 // SuperLU direct solve, CuPy CG+Jacobi) at its call site
@@ -124,7 +124,10 @@ __global__ __launch_bounds__(kBlock) void k_zero_fixed(int nnode, const uint8_t*
 }
 
 // ---- mixed precision: FP64 flexible CG around an FP32 multigrid correction ----
-// r32 = r64 / ||r64||  (||r||^2 = sc->rr, the current CG residual)
+// The FP32 preconditioner sees r / zs and its output is scaled back by zs, with
+// zs a recent residual norm (sc->zs), which keeps the FP32 values near unit
+// size whatever the load magnitude.
+// r32 = r64 / zs (first correction: zs = ||r0||)
 __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __restrict__ r64,
                                                          float* __restrict__ r32,
                                                          const int* __restrict__ done,
@@ -132,19 +135,43 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __r
   if (*done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double rr = sc->rr;
-  const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
+  const double zs = sc->zs;
+  const double inv = zs > 0.0 ? 1.0 / zs : 0.0;
   r32[i] = (float)(r64[i] * inv);
 }
 
-// z = ||r|| z32; partials of r.z and r.z_old (kFinRzFlex: Polak-Ribiere beta)
+// x += alpha p; r -= alpha q; r32 = r / ||r_old|| (the norm before this
+// update; kFinUpdate records it as zs); r.r partials.
+__global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __restrict__ x,
+                                                            double* __restrict__ r,
+                                                            const double* __restrict__ p,
+                                                            const double* __restrict__ q,
+                                                            float* __restrict__ r32,
+                                                            const int* __restrict__ done,
+                                                            double* partials, const Scalars* sc) {
+  if (*done) return;
+  const double alpha = sc->alpha;
+  const double rr = sc->rr;
+  const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    r32[i] = (float)(ri * inv);
+    v[0] += ri * ri;
+  }
+  block_partials<1>(v, partials);
+}
+
+// z = zs z32; partials of r.z and r.z_old (kFinRzFlex: Polak-Ribiere beta)
 __global__ __launch_bounds__(kBlock) void k_cg_rz_flex(int n, const double* __restrict__ r,
                                                         const float* __restrict__ z32,
                                                         const double* __restrict__ zold,
                                                         int first, const int* __restrict__ done,
                                                         double* partials, const Scalars* sc) {
   if (*done) return;
-  const double nrm = sqrt(sc->rr);
+  const double nrm = sc->zs;
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double z = nrm * (double)z32[i];
@@ -163,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir_flex(int n, const float* __re
   if (*done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double z = sqrt(sc->rr) * (double)z32[i];
+  const double z = sc->zs * (double)z32[i];
   p[i] = first ? z : z + sc->beta * p[i];
   zold[i] = z;
 }
@@ -189,12 +216,27 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
-  launch_cheb_first<T>(P, L, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta, x_zero, s);
-  std::swap(x, xa);
-  for (int k = 1; k < P.opt.smooth_degree; ++k) {
+  int k = 1;
+  if (x_zero && P.use_el && P.opt.smooth_degree >= 2) {
+    // steps 1 and 2 from x = 0 in one sweep
+    const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+    if (&L == &P.lv[0]) {
+      if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
+    }
+    el_cheb_zero2<T>(P, L, B.s, B.dinv, b, xa, B.d, 1.0 / theta, rho_new * rho_old,
+                     2.0 * rho_new / delta, P.opt.smooth_degree > 2, &P.sc->done, s);
+    std::swap(x, xa);
+    rho_old = rho_new;
+    k = 2;
+  } else {
+    launch_cheb_first<T>(P, L, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta, x_zero,
+                         P.opt.smooth_degree > 1, s);
+    std::swap(x, xa);
+  }
+  for (; k < P.opt.smooth_degree; ++k) {
     const double rho_new = 1.0 / (2.0 * sigma - rho_old);
     launch_cheb_step<T>(P, L, B.s, B.dinv, b, x, xa, B.d, rho_new * rho_old, 2.0 * rho_new / delta,
-                        s);
+                        k + 1 < P.opt.smooth_degree, s);
     std::swap(x, xa);
     rho_old = rho_new;
   }
@@ -314,7 +356,9 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
                      P.partials);
   finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
   auto correction = [&](int first) {
-    hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done, P.sc);
+    if (first)
+      hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done,
+                         P.sc);
     const float* z32 = precondition<float>(P, r32, P.q32, jacobi, s);
     hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z32, zold,
                        first, done, P.partials, P.sc);
@@ -329,8 +373,8 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
     const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
     for (int k = 0; k < m; ++k) {
       launch_apply<double>(P, L0, L0.f64.s, p, q, nullptr, 1, s);
-      hipLaunchKernelGGL(k_cg_update<double>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p,
-                         q, done, P.partials, P.sc);
+      hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p, q,
+                         r32, done, P.partials, P.sc);
       finalize(P, kFinUpdate, red_blocks(n), s);
       correction(0);
     }
@@ -441,10 +485,10 @@ int smooth_step_fine(Plan& P, int prec, const void* b, const void* x, void* xo, 
   }
   if (prec == 0)
     el_cheb<double>(P, L, L.f64.s, L.f64.dinv, (const double*)b, (const double*)x, (double*)xo,
-                    (double*)d, 0.0, c1, c2, 0, nullptr, s);
+                    (double*)d, 0.0, c1, c2, 0, 1, nullptr, s);
   else
     el_cheb<float>(P, L, L.f32.s, L.f32.dinv, (const float*)b, (const float*)x, (float*)xo,
-                   (float*)d, 0.0, c1, c2, 0, nullptr, s);
+                   (float*)d, 0.0, c1, c2, 0, 1, nullptr, s);
   TK_HIP(hipGetLastError());
   return 0;
 }
