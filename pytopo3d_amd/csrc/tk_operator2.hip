@@ -57,12 +57,28 @@ struct Epilogue {
                    // element scales (KE's diagonal is the same at every corner)
   T c0, c1, c2;
   T kd[3];         // KE diagonal per component
+  // MODE 5 (flexible CG direction): p = zs z32 + beta p32o, stored as p32n
+  const float* z32;
+  const float* p32o;
+  float* p32n;
+  const Scalars* sc;
 };
 
 // kModeChebZero fuses the first two Chebyshev steps from x = 0: the node
 // values are formed on the fly as x1 = c0 D^-1 b (= d1), then
 // d2 = c1 d1 + c2 D^-1 (b - K x1), x2 = x1 + d2.
-enum { kModeApply = 0, kModeResidual = 1, kModeChebFirst = 2, kModeChebStep = 3, kModeChebZero = 4 };
+// kModeApplyDir (FP64 only) fuses the flexible-CG direction update into the
+// operator: the node operand is p = zs z32 + beta p32o rounded to FP32 (the
+// direction is stored in FP32; K p and p.Kp are FP64), written to p32n by the
+// node owners; y = K p and the partials of p.Kp as in kModeApply.
+enum {
+  kModeApply = 0,
+  kModeResidual = 1,
+  kModeChebFirst = 2,
+  kModeChebStep = 3,
+  kModeChebZero = 4,
+  kModeApplyDir = 5
+};
 
 constexpr int kTile = 16;          // threads per tile edge
 constexpr int kOut = kTile - 1;    // owned nodes per tile edge
@@ -207,6 +223,20 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
   const int xny = Y0 - 1 + xi, xnx = X0 - 1 + xj;
   const bool ext_node = xi >= 0 && (unsigned)xny < (unsigned)g.nny && (unsigned)xnx < (unsigned)g.nnx;
 
+  // kModeApplyDir: direction value at DOF 3n + c (first iteration: beta = 0
+  // and p32o is not read)
+  // Without a previous direction the second operand is z32 itself with a
+  // zero coefficient (finite whenever z32 is), so the loads stay unconditional.
+  T dir_zs = T(0), dir_beta = T(0);
+  const float* __restrict__ dir_p = ep.p32o ? ep.p32o : ep.z32;
+  if constexpr (MODE == kModeApplyDir) {
+    dir_zs = (T)ep.sc->zs;
+    dir_beta = ep.p32o ? (T)ep.sc->beta : T(0);
+  }
+  auto dir_value = [&](int n, int c) -> T {
+    const T z = (T)ep.z32[3 * n + c], po = (T)dir_p[3 * n + c];
+    return (T)(float)(dir_zs * z + dir_beta * po);
+  };
   // global loads of node plane pz (registers only)
   auto fetch = [&](int pz, PlaneRegs<T, MODE>& R) {
 #pragma unroll
@@ -221,6 +251,8 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
           R.ob[c] = ep.b[3 * n + c];
           R.odi[c] = ep.dinv[3 * n + c];
           R.own[c] = ep.c0 * R.odi[c] * R.ob[c];
+        } else if constexpr (MODE == kModeApplyDir) {
+          R.own[c] = dir_value(n, c);
         } else {
           R.own[c] = u[3 * n + c];
         }
@@ -234,6 +266,7 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
       for (int c = 0; c < 3; ++c) {
         T v;
         if constexpr (MODE == kModeChebZero) v = ep.c0 * ep.dinv[3 * n + c] * ep.b[3 * n + c];
+        else if constexpr (MODE == kModeApplyDir) v = dir_value(n, c);
         else v = u[3 * n + c];
         R.ext[c] = ((f >> c) & 1) ? T(0) : v;
       }
@@ -361,9 +394,10 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
         const int i = 3 * n_out + c;
         const bool fx = (fix_prev >> c) & 1;
         const T val = fx ? raw_prev[c] : ku;
-        if (MODE == kModeApply) {
+        if (MODE == kModeApply || MODE == kModeApplyDir) {
           ep.y[i] = val;
           dot += (double)raw_prev[c] * (double)val;
+          if (MODE == kModeApplyDir) ep.p32n[i] = (float)raw_prev[c];
         } else if (MODE == kModeResidual) {
           // zero on fixed DOFs: the restriction reads no fine mask
           ep.y[i] = fx ? T(0) : eb[c] - val;
@@ -398,7 +432,7 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
     if (ez + 1 >= Z1) break;
     layer(ez + 1, 0, F1, F0, RB, RA);
   }
-  if (MODE == kModeApply && with_dot) {
+  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot) {
     double v[1] = {dot};
     block_partials<1>(v, partials);
   }
@@ -515,7 +549,24 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
   for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
   hipLaunchKernelGGL((k_apply_el<T, MODE>), grid, dim3(256), 0, s, L.g, kv, sectors_of<T>(P), sl,
                      L.fixed, u, ep, nty, zc, with_dot, done, P.partials);
-  if (MODE == kModeApply && with_dot) finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
+  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot)
+    finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
+}
+
+// Flexible-CG direction and operator in one sweep (see kModeApplyDir):
+// p32n = fp32(zs z32 + beta p32o) (p32o == nullptr: beta = 0), y = K p, and
+// the p.Kp partials finalised into alpha.
+void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, double* y,
+                  hipStream_t s) {
+  const Level& L = P.lv[0];
+  P.cnt.applies64++;
+  Epilogue<double> ep{};
+  ep.y = y;
+  ep.z32 = z32;
+  ep.p32o = p32o;
+  ep.p32n = p32n;
+  ep.sc = P.sc;
+  launch_el<double, kModeApplyDir>(P, L, L.f64.s, nullptr, ep, 1, &P.sc->done, s);
 }
 
 template <typename T>

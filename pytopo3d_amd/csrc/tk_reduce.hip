@@ -69,8 +69,8 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       sc->rz = s[0];
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;
-    case kFinRzFlex:  // s0 = r.z, s1 = r.z_old (Polak-Ribiere)
-      sc->beta = i0 ? 0.0 : (s[0] - s[1]) / sc->rz;
+    case kFinRzFlex:  // s0 = r.z, s1 = z.K p_old: beta = -z.K p_old / p_old.K p_old
+      sc->beta = i0 ? 0.0 : -s[1] / sc->pq;
       sc->rz = s[0];
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;

@@ -140,11 +140,11 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __r
   r32[i] = (float)(r64[i] * inv);
 }
 
-// x += alpha p; r -= alpha q; r32 = r / ||r_old|| (the norm before this
-// update; kFinUpdate records it as zs); r.r partials.
+// x += alpha p (p stored in FP32); r -= alpha q; r32 = r / ||r_old|| (the norm
+// before this update; kFinUpdate records it as zs); r.r partials.
 __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __restrict__ x,
                                                             double* __restrict__ r,
-                                                            const double* __restrict__ p,
+                                                            const float* __restrict__ p32,
                                                             const double* __restrict__ q,
                                                             float* __restrict__ r32,
                                                             const int* __restrict__ done,
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __rest
   const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
   double v[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
+    x[i] += alpha * (double)p32[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     r32[i] = (float)(ri * inv);
@@ -164,10 +164,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __rest
   block_partials<1>(v, partials);
 }
 
-// z = zs z32; partials of r.z and r.z_old (kFinRzFlex: Polak-Ribiere beta)
+// z = zs z32; partials of r.z and q.z, q = K p_old (kFinRzFlex:
+// beta = -(z.K p_old) / (p_old.K p_old), the flexible-CG orthogonalisation
+// against the previous direction)
 __global__ __launch_bounds__(kBlock) void k_cg_rz_flex(int n, const double* __restrict__ r,
+                                                        const double* __restrict__ q,
                                                         const float* __restrict__ z32,
-                                                        const double* __restrict__ zold,
                                                         int first, const int* __restrict__ done,
                                                         double* partials, const Scalars* sc) {
   if (*done) return;
@@ -176,23 +178,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_rz_flex(int n, const double* __re
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double z = nrm * (double)z32[i];
     v[0] += r[i] * z;
-    if (!first) v[1] += r[i] * zold[i];
+    if (!first) v[1] += q[i] * z;
   }
   block_partials<2>(v, partials);
 }
 
-// p = z + beta p, z_old = z
+// p32n = fp32(zs z32 + beta p32o) and its FP64 copy (operators without the
+// fused direction update); p32o == nullptr: beta = 0
 __global__ __launch_bounds__(kBlock) void k_cg_dir_flex(int n, const float* __restrict__ z32,
-                                                         double* __restrict__ zold,
-                                                         double* __restrict__ p, int first,
+                                                         const float* __restrict__ p32o,
+                                                         float* __restrict__ p32n,
+                                                         double* __restrict__ p64,
                                                          const int* __restrict__ done,
                                                          const Scalars* __restrict__ sc) {
   if (*done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double z = sc->zs * (double)z32[i];
-  p[i] = first ? z : z + sc->beta * p[i];
-  zold[i] = z;
+  double v = sc->zs * (double)z32[i];
+  if (p32o) v += sc->beta * (double)p32o[i];
+  const float pf = (float)v;
+  p32n[i] = pf;
+  p64[i] = (double)pf;
 }
 
 // ------------------------------------------------------------------ helpers
@@ -340,43 +346,53 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
 }
 
 // FP64 flexible PCG whose preconditioner is the FP32 V-cycle (or FP32 Jacobi)
-// applied to the normalised FP64 residual: every correction is computed in
-// FP32, while the residual recurrence, dot products and step lengths stay FP64,
-// so the iteration reaches FP64 accuracy whatever the stiffness contrast.
+// applied to the scaled FP64 residual: every correction is computed in FP32,
+// the search direction is stored in FP32, while q = K p, the residual
+// recurrence, dot products and step lengths stay FP64, so the iteration
+// reaches FP64 accuracy whatever the stiffness contrast.  Directions are
+// orthogonalised against the previous one only (flexible CG with truncation 1).
 static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rtol, int maxit,
                      int jacobi, hipStream_t s) {
   const int n = P.g.ndof;
   Level& L0 = P.lv[0];
   const int* done = &P.sc->done;
-  double *r = P.r64, *p = P.p64, *q = P.q64, *zold = P.x64;
+  double *r = P.r64, *q = P.q64, *p64 = P.p64;
+  float* pbuf[2] = {P.p32, P.x32};   // direction, double-buffered (read old, write new)
   float* r32 = L0.f32.b;
   if (x_zero) hipLaunchKernelGGL(k_copy_zero<double>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, b, r, x);
   else launch_residual<double>(P, L0, L0.f64.s, b, x, r, s, nullptr);
   hipLaunchKernelGGL(k_cg_norms<double>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, b, r,
                      P.partials);
   finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
-  auto correction = [&](int first) {
-    if (first)
-      hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done,
-                         P.sc);
+  hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done, P.sc);
+  auto correction = [&](int first) -> const float* {
     const float* z32 = precondition<float>(P, r32, P.q32, jacobi, s);
-    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z32, zold,
-                       first, done, P.partials, P.sc);
+    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, q, z32, first,
+                       done, P.partials, P.sc);
     finalize(P, kFinRzFlex, red_blocks(n), s, 0.0, first);
-    hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z32, zold, p, first,
-                       done, P.sc);
+    return z32;
   };
-  correction(1);
-  int it = 0;
+  const float* z32 = correction(1);
+  int it = 0, cur = 0;
+  const float* pold = nullptr;   // no previous direction before the first step
   const int chunk = P.opt.check_every > 0 ? P.opt.check_every : 8;
   while (it < maxit) {
     const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
     for (int k = 0; k < m; ++k) {
-      launch_apply<double>(P, L0, L0.f64.s, p, q, nullptr, 1, s);
-      hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p, q,
-                         r32, done, P.partials, P.sc);
+      float* pnew = pbuf[cur];
+      if (P.use_el) {
+        el_apply_dir(P, z32, pold, pnew, q, s);
+      } else {
+        hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z32, pold,
+                           pnew, p64, done, P.sc);
+        launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, s);
+      }
+      hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, pnew,
+                         q, r32, done, P.partials, P.sc);
       finalize(P, kFinUpdate, red_blocks(n), s);
-      correction(0);
+      z32 = correction(0);
+      pold = pnew;
+      cur ^= 1;
     }
     it += m;
     if (read_scalars(P, s)) return 2;
