@@ -55,11 +55,12 @@ typedef struct tk_options {
   double cheb_hi;         /* (1.0) */
   int max_levels;         /* multigrid levels (12) */
   int coarse_max_dof;     /* coarsest grid DOF limit, <= 192 (192) */
-  double inner_rtol;      /* mixed precision: inner FP32 tolerance (1e-4) */
-  int inner_maxit;        /* mixed precision: inner iteration cap per refinement step (200) */
   int check_every;        /* iterations between host convergence checks (8) */
   int operator_kind;      /* 0: element kernel in the Walsh-Hadamard sector form when KE is
                              mirror-symmetric (default), 1: dense node-gather kernel */
+  int no_graphs;          /* 0: each chunk of check_every Krylov iterations is captured once
+                             per plan as a HIP graph and replayed (default); 1: kernels are
+                             launched one by one */
 } tk_options;
 
 /* Fill *opt with the defaults. */

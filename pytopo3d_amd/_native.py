@@ -39,10 +39,9 @@ class TkOptions(ctypes.Structure):
         ("cheb_hi", ctypes.c_double),
         ("max_levels", ctypes.c_int),
         ("coarse_max_dof", ctypes.c_int),
-        ("inner_rtol", ctypes.c_double),
-        ("inner_maxit", ctypes.c_int),
         ("check_every", ctypes.c_int),
         ("operator_kind", ctypes.c_int),
+        ("no_graphs", ctypes.c_int),
     ]
 
 

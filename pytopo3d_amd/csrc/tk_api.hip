@@ -81,6 +81,13 @@ static void free_plan(Plan* P) {
   f(P->ework0); f(P->ework1); f(P->ework2); f(P->ework3);
   f(P->partials); f(P->counter); f(P->sc);
   if (P->sc_host) (void)hipHostFree(P->sc_host);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < Plan::kChunkSizes; ++c)
+        if (P->gexec[a][b][c]) (void)hipGraphExecDestroy(P->gexec[a][b][c]);
+  if (P->gev_in) (void)hipEventDestroy(P->gev_in);
+  if (P->gev_out) (void)hipEventDestroy(P->gev_out);
+  if (P->gstream) (void)hipStreamDestroy(P->gstream);
   delete P;
 }
 
@@ -132,8 +139,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->max_levels > 0) P->opt.max_levels = o->max_levels < kMaxLevels ? o->max_levels : kMaxLevels;
     if (o->coarse_max_dof > 0)
       P->opt.coarse_max_dof = o->coarse_max_dof < kCoarseMaxDof ? o->coarse_max_dof : kCoarseMaxDof;
-    if (o->inner_rtol > 0.0) P->opt.inner_rtol = o->inner_rtol;
-    if (o->inner_maxit > 0) P->opt.inner_maxit = o->inner_maxit;
+    if (o->no_graphs == 0 || o->no_graphs == 1) P->opt.no_graphs = o->no_graphs;
     if (o->check_every > 0) P->opt.check_every = o->check_every;
     if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
   }
@@ -327,10 +333,9 @@ void tk_options_default(tk_options* o) {
   o->cheb_hi = d.cheb_hi;
   o->max_levels = d.max_levels;
   o->coarse_max_dof = d.coarse_max_dof;
-  o->inner_rtol = d.inner_rtol;
-  o->inner_maxit = d.inner_maxit;
   o->check_every = d.check_every;
   o->operator_kind = d.operator_kind;
+  o->no_graphs = d.no_graphs;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

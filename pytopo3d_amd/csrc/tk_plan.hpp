@@ -34,10 +34,9 @@ struct Options {
   double cheb_hi = 1.0;
   int max_levels = kMaxLevels;
   int coarse_max_dof = kCoarseMaxDof;
-  double inner_rtol = 1e-4;      // mixed precision: inner FP32 solve tolerance
-  int inner_maxit = 200;
   int check_every = 8;           // Krylov convergence is read back every N iterations
   int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node kernel
+  int no_graphs = 0;             // 1: no captured solver chunks
 };
 
 struct Counters {
@@ -99,6 +98,14 @@ struct Plan {
   // bookkeeping of the last solve
   double last_penal = -1.0;
   int last_prec = 0;
+  // captured Krylov chunks, [precision][jacobi]: every launch argument of a
+  // chunk is plan-owned (the iterate lives in x64), so a chunk is captured
+  // once per plan and replayed on gstream.
+  hipStream_t gstream = nullptr;
+  hipEvent_t gev_in = nullptr, gev_out = nullptr;
+  static constexpr int kChunkSizes = 4;   // chunk lengths 2 * base, base, base / 2, 2
+  hipGraphExec_t gexec[2][2][kChunkSizes] = {};
+  Counters gdelta[2][2][kChunkSizes];      // host counters added by one replay
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
 };
 

@@ -295,6 +295,110 @@ static const T* precondition(Plan& P, const T* r, T* zbuf, int jacobi, hipStream
   return vcycle<T>(P, r, s);
 }
 
+// ------------------------------------------------------------------ chunk runner
+static void add_counters(Counters& a, const Counters& d, int sign) {
+  a.applies64 += sign * d.applies64;
+  a.applies32 += sign * d.applies32;
+  a.cg_iters += sign * d.cg_iters;
+  a.ir_steps += sign * d.ir_steps;
+  a.solves += sign * d.solves;
+  a.oc_steps += sign * d.oc_steps;
+  a.filter_passes += sign * d.filter_passes;
+}
+
+// Runs `body(stream, m)` -- m Krylov iterations whose launches all return at
+// once after the device-side `done` flag is set -- in chunks until the flag is
+// set, reading the device scalars back after every chunk.  Chunk lengths are
+// taken from {2 base, base, base / 2, 2} (base = check_every rounded up to an
+// even count): the next length is the largest one not above the iterations
+// still needed, predicted from the residual decay of the previous chunk, so
+// few iterations run past convergence.  With graphs enabled every chunk
+// length is captured on the plan's stream the first time it is used and
+// replayed from then on (the kernels, their order and arguments are the same
+// in every chunk of that length), so the device runs it without host launch
+// gaps.
+template <class Body>
+static int run_chunks(Plan& P, int prec, int jacobi, int maxit, Body body, hipStream_t s) {
+  bool graphs = !P.opt.no_graphs;
+  if (graphs && !P.gstream) {
+    if (hipStreamCreateWithFlags(&P.gstream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&P.gev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&P.gev_out, hipEventDisableTiming) != hipSuccess) {
+      set_error("could not create the solver stream");
+      return 2;
+    }
+  }
+  int base = P.opt.check_every > 0 ? P.opt.check_every : 8;
+  base += base & 1;
+  int sizes[Plan::kChunkSizes] = {2 * base, base, base / 2 + ((base / 2) & 1), 2};
+  const int pa = prec ? 1 : 0, ja = jacobi ? 1 : 0;
+  // capture every chunk length at the first use, so later solves never pay it
+  auto capture = [&](int k) -> int {
+    const int m = sizes[k];
+    hipGraphExec_t& ex = P.gexec[pa][ja][k];
+    Counters& delta = P.gdelta[pa][ja][k];
+    if (!ex) {
+      const Counters before = P.cnt;
+      TK_HIP(hipStreamBeginCapture(P.gstream, hipStreamCaptureModeThreadLocal));
+      body(P.gstream, m);
+      hipGraph_t g = nullptr;
+      TK_HIP(hipStreamEndCapture(P.gstream, &g));
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ex = nullptr;
+        set_error(std::string("HIP error: ") + hipGetErrorString(e) + " (graph instantiation)");
+        return 2;
+      }
+      delta = P.cnt;
+      add_counters(delta, before, -1);
+      P.cnt = before;
+    }
+    return 0;
+  };
+  if (graphs)
+    for (int k = 0; k < Plan::kChunkSizes; ++k)
+      if (int rc = capture(k)) return rc;
+  auto run = [&](int k) -> int {
+    if (!graphs) {
+      body(s, sizes[k]);
+      return 0;
+    }
+    hipGraphExec_t& ex = P.gexec[pa][ja][k];
+    Counters& delta = P.gdelta[pa][ja][k];
+    TK_HIP(hipEventRecord(P.gev_in, s));
+    TK_HIP(hipStreamWaitEvent(P.gstream, P.gev_in, 0));
+    TK_HIP(hipGraphLaunch(ex, P.gstream));
+    TK_HIP(hipEventRecord(P.gev_out, P.gstream));
+    TK_HIP(hipStreamWaitEvent(s, P.gev_out, 0));
+    add_counters(P.cnt, delta, 1);
+    return 0;
+  };
+  int k = 1;                       // first chunk: base iterations
+  int it_prev = 0;
+  double rr_prev = 0.0;
+  for (long long guard = 0; guard < (long long)maxit + 8; ++guard) {
+    if (int rc = run(k)) return rc;
+    if (read_scalars(P, s)) return 2;
+    const Scalars& H = *P.sc_host;
+    if (H.done) break;
+    // iterations still needed at the observed residual decay rate
+    const double target = H.tol2 * H.bb;
+    int need = sizes[0];
+    if (it_prev < H.iter && rr_prev > 0.0 && H.rr > 0.0 && H.rr < rr_prev && target > 0.0) {
+      const double per_it = std::log(H.rr / rr_prev) / (double)(H.iter - it_prev);
+      const double est = std::log(target / H.rr) / per_it;
+      if (est < (double)sizes[0]) need = est < 1.0 ? 1 : (int)std::ceil(est);
+    }
+    it_prev = H.iter;
+    rr_prev = H.rr;
+    k = Plan::kChunkSizes - 1;
+    for (int j = 0; j < Plan::kChunkSizes; ++j)
+      if (sizes[j] <= need) { k = j; break; }
+  }
+  return 0;
+}
+
 // PCG on K x = b (fine level, precision T).  x is in/out (x_zero: start at 0).
 // Returns 0, or 2 on a HIP error.  Iterations and the final recurrence
 // residual are left in P.sc_host.
@@ -317,27 +421,23 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z, done, P.partials);
   finalize(P, kFinRz, red_blocks(n), s, 0.0, 1);
   hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, p, 1, done, P.sc);
-  int it = 0;
-  const int chunk = P.opt.check_every > 0 ? P.opt.check_every : 8;
-  while (it < maxit) {
-    const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
+  auto body = [&](hipStream_t st, int m) {
     for (int k = 0; k < m; ++k) {
-      launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, s);
-      hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, p, q,
+      launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, st);
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, p, q,
                          done, P.partials, P.sc);
-      finalize(P, kFinUpdate, red_blocks(n), s);
+      finalize(P, kFinUpdate, red_blocks(n), st);
       // the preconditioner output must not alias q (read by nothing after the update)
-      const T* zz = precondition<T>(P, r, q, jacobi, s);
-      hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, zz, done,
+      const T* zz = precondition<T>(P, r, q, jacobi, st);
+      hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, zz, done,
                          P.partials);
-      finalize(P, kFinRz, red_blocks(n), s, 0.0, 0);
-      hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, zz, p, 0, done,
+      finalize(P, kFinRz, red_blocks(n), st, 0.0, 0);
+      hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, zz, p, 0, done,
                          P.sc);
     }
-    it += m;
-    if (read_scalars(P, s)) return 2;
-    if (P.sc_host->done) break;
-  }
+  };
+  int rc = run_chunks(P, 0, jacobi, maxit, body, s);
+  if (rc) return rc;
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in pcg");
     return 2;
@@ -365,39 +465,36 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
                      P.partials);
   finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
   hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done, P.sc);
-  auto correction = [&](int first) -> const float* {
-    const float* z32 = precondition<float>(P, r32, P.q32, jacobi, s);
-    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, q, z32, first,
+  auto correction = [&](int first, hipStream_t st) -> const float* {
+    const float* z = precondition<float>(P, r32, P.q32, jacobi, st);
+    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, q, z, first,
                        done, P.partials, P.sc);
-    finalize(P, kFinRzFlex, red_blocks(n), s, 0.0, first);
-    return z32;
+    finalize(P, kFinRzFlex, red_blocks(n), st, 0.0, first);
+    return z;
   };
-  const float* z32 = correction(1);
-  int it = 0, cur = 0;
-  const float* pold = nullptr;   // no previous direction before the first step
-  const int chunk = P.opt.check_every > 0 ? P.opt.check_every : 8;
-  while (it < maxit) {
-    const int m = (maxit - it < chunk) ? (maxit - it) : chunk;
-    for (int k = 0; k < m; ++k) {
-      float* pnew = pbuf[cur];
-      if (P.use_el) {
-        el_apply_dir(P, z32, pold, pnew, q, s);
-      } else {
-        hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z32, pold,
-                           pnew, p64, done, P.sc);
-        launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, s);
-      }
-      hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, x, r, pnew,
-                         q, r32, done, P.partials, P.sc);
-      finalize(P, kFinUpdate, red_blocks(n), s);
-      z32 = correction(0);
-      pold = pnew;
-      cur ^= 1;
+  const float* z32 = correction(1, s);
+  // iteration i writes direction pbuf[i & 1] and reads pbuf[(i - 1) & 1]
+  // (none for i = 0); chunks of even length start at odd i, so every chunk
+  // launches the same kernels with the same arguments.
+  auto iteration = [&](float* pnew, const float* pold, hipStream_t st) {
+    if (P.use_el) {
+      el_apply_dir(P, z32, pold, pnew, q, st);
+    } else {
+      hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, st, n, z32, pold, pnew,
+                         p64, done, P.sc);
+      launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, st);
     }
-    it += m;
-    if (read_scalars(P, s)) return 2;
-    if (P.sc_host->done) break;
-  }
+    hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, pnew,
+                       q, r32, done, P.partials, P.sc);
+    finalize(P, kFinUpdate, red_blocks(n), st);
+    z32 = correction(0, st);
+  };
+  iteration(pbuf[0], nullptr, s);
+  auto body = [&](hipStream_t st, int m) {   // m is even
+    for (int k = 0; k < m; ++k) iteration(pbuf[(k + 1) & 1], pbuf[k & 1], st);
+  };
+  int rc = run_chunks(P, 1, jacobi, maxit, body, s);
+  if (rc) return rc;
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in mixed pcg");
     return 2;
@@ -442,14 +539,21 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
   double* b = P.lv[0].f64.b;  // masked right-hand side
   hipLaunchKernelGGL(k_mask_rhs, dim3(nblocks(P.g.nnode)), dim3(kBlock), 0, s, P.g.nnode,
                      P.lv[0].fixed, F, b);
-  if (use_x0)
+  // the iterate lives in the plan (x64) so that captured chunks stay valid
+  // for every caller buffer; U is copied in (warm start) and out
+  double* x = P.x64;
+  if (use_x0) {
+    TK_HIP(hipMemcpyAsync(x, U, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_zero_fixed, dim3(nblocks(P.g.nnode)), dim3(kBlock), 0, s, P.g.nnode,
-                       P.lv[0].fixed, U);
+                       P.lv[0].fixed, x);
+  }
   int total_it = 0;
   double relres = 0.0;
+  int rc = (prec == 0) ? pcg<double>(P, b, x, P.r64, P.p64, P.q64, !use_x0, rtol, maxit, jacobi, s)
+                       : pcg_mixed(P, b, x, !use_x0, rtol, maxit, jacobi, s);
+  if (rc) return rc;
+  TK_HIP(hipMemcpyAsync(U, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   if (prec == 0) {
-    int rc = pcg<double>(P, b, U, P.r64, P.p64, P.q64, !use_x0, rtol, maxit, jacobi, s);
-    if (rc) return rc;
     total_it = P.sc_host->iter;
     relres = (P.sc_host->bb > 0.0) ? std::sqrt(P.sc_host->rr / P.sc_host->bb) : 0.0;
     P.cnt.cg_iters += total_it;
@@ -460,8 +564,6 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
       return 3;
     }
   } else {
-    int rc = pcg_mixed(P, b, U, !use_x0, rtol, maxit, jacobi, s);
-    if (rc) return rc;
     total_it = P.sc_host->iter;
     relres = (P.sc_host->bb > 0.0) ? std::sqrt(P.sc_host->rr / P.sc_host->bb) : 0.0;
     P.cnt.ir_steps += total_it;
