@@ -610,8 +610,7 @@ int tk_oc_update(tk_plan* plan, const double* x, const double* dc_dx, const doub
   tk::launch_oc_normalize(P, P.ework0, s);
   tk::launch_oc_init(P, target, max_steps, tol, s);
   for (int k = 0; k < max_steps; ++k) {
-    tk::launch_oc_candidate(P, x, P.ework0, move, 0, P.ework1, s);
-    tk::launch_oc_volume(P, P.ework1, beta, eta, tb, den, s);
+    tk::launch_oc_step(P, x, P.ework0, move, P.ework1, beta, eta, tb, den, s);
   }
   tk::launch_oc_candidate(P, x, P.ework0, move, 1, x_new, s);
   TK_LAUNCH_CHECK();

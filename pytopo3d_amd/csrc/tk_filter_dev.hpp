@@ -17,6 +17,19 @@ __device__ __forceinline__ void decode_offset(int o, int& dy, int& dx, int& dz) 
   dz = ((o >> 16) & 255) - 16;
 }
 
+// OC candidate of one element at multiplier lam (pytopo3d/utils/oc_update.py:
+// 25-124): clip(max(x - m, min(x + m, x sqrt(max(ratio / (lam + 1e-30),
+// 1e-30)))), 0, 1) on free elements, 1 / 0 on protected solid / void.  Every
+// kernel that forms candidates calls this, so they agree bitwise.
+__device__ __forceinline__ double oc_candidate(uint8_t f, double xe, double ratio, double lam,
+                                               double move) {
+  if (f != 0) return (f & kVoid) ? 0.0 : 1.0;
+  const double r = fmax(ratio / (lam + 1e-30), 1e-30);
+  double c = xe * sqrt(r);
+  c = fmax(xe - move, fmin(xe + move, c));
+  return fmin(fmax(c, 0.0), 1.0);
+}
+
 // Stage the stencil in LDS; all threads of the block must call.
 __device__ __forceinline__ void stage_stencil(int nsten, const int* __restrict__ off,
                                               const double* __restrict__ w, int* s_off,

@@ -8,8 +8,8 @@
 //   filtered candidate; the lambda with the smallest |V - target| (strict <)
 //   is kept; V > target raises the lower bound; stop when
 //   (hi - lo) / max(hi + lo, tiny) < tol or after max_steps.
-// The kept candidate is rebuilt from the kept lambda by the same kernel, so it
-// is bitwise the candidate that was measured.
+// The kept candidate is rebuilt from the kept lambda with the same formula
+// (oc_candidate), so it is bitwise the candidate that was measured.
 // Also: change / volume / gray statistics (pytopo3d/core/optimizer.py:642-655).
 #include "tk_filter_dev.hpp"
 #include "tk_plan.hpp"
@@ -66,20 +66,9 @@ __global__ __launch_bounds__(kBlock) void k_oc_candidate(int n, const uint8_t* _
   if (!use_best && sc->oc_done) return;
   const double lam = use_best ? sc->best_lam : 0.5 * (sc->lo + sc->hi);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e == 0 && !use_best) sc->lam = lam;
   if (e >= n) return;
   const uint8_t f = fl[e];
-  double v;
-  if (f == 0) {
-    const double xe = x[e];
-    const double r = fmax(ratio[e] / (lam + 1e-30), 1e-30);
-    double c = xe * sqrt(r);
-    c = fmax(xe - move, fmin(xe + move, c));
-    v = fmin(fmax(c, 0.0), 1.0);
-  } else {
-    v = (f & kVoid) ? 0.0 : 1.0;
-  }
-  cand[e] = v;
+  cand[e] = oc_candidate(f, f == 0 ? x[e] : 0.0, f == 0 ? ratio[e] : 0.0, lam, move);
 }
 
 __global__ __launch_bounds__(kBlock) void k_oc_volume(Grid g, int nsten, const int* __restrict__ off,
@@ -155,13 +144,17 @@ void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double m
                      P.eflags, x, ratio, move, use_best, cand, P.sc);
 }
 
-void launch_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb_eta,
-                      double den, hipStream_t s) {
+// One bisection step at lambda = (lo + hi) / 2: the candidate (formed inside
+// the tiled filter, or by k_oc_candidate into cand for the gather filter), its
+// free-element projected volume, and the bracket update (kFinOcVolume).
+void launch_oc_step(Plan& P, const double* x, const double* ratio, double move, double* cand,
+                    double beta, double eta, double tb_eta, double den, hipStream_t s) {
   P.cnt.oc_steps++;
   if (P.ftiled) {
-    tiled_oc_volume(P, cand, beta, eta, tb_eta, den, s);
+    tiled_oc_volume(P, x, ratio, move, beta, eta, tb_eta, den, s);
     return;
   }
+  launch_oc_candidate(P, x, ratio, move, 0, cand, s);
   hipLaunchKernelGGL(k_oc_volume, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.nsten,
                      P.sten_off, P.sten_w, P.Hs, P.eflags, cand, beta, eta, tb_eta, den,
                      P.partials, P.counter, P.sc);

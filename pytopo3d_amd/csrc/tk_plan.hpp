@@ -169,8 +169,8 @@ void tiled_filter_project(Plan& P, const double* x, double beta, double eta, dou
 void tiled_filter_plain(Plan& P, const double* v, double* out, hipStream_t s);
 void tiled_free_sum(Plan& P, const double* x, double beta, double eta, double tb, double den,
                     int slot, hipStream_t s);
-void tiled_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb, double den,
-                     hipStream_t s);
+void tiled_oc_volume(Plan& P, const double* x, const double* ratio, double move, double beta,
+                     double eta, double tb, double den, hipStream_t s);
 template <typename T>
 void el_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, int with_dot,
               const int* done, hipStream_t s);
@@ -197,8 +197,8 @@ void launch_oc_ratio(Plan& P, const double* dc, const double* dv, double* ratio,
 void launch_oc_normalize(Plan& P, double* ratio, hipStream_t s);
 void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double move, int use_best,
                          double* cand, hipStream_t s);
-void launch_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb_eta,
-                      double den, hipStream_t s);
+void launch_oc_step(Plan& P, const double* x, const double* ratio, double move, double* cand,
+                    double beta, double eta, double tb_eta, double den, hipStream_t s);
 void launch_design_stats(Plan& P, const double* x_old, const double* x_new, const double* rho,
                          hipStream_t s);
 

@@ -75,9 +75,10 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;
     case kFinRatioMax: sc->ratio_scale = m; break;
-    case kFinOcVolume: {
+    case kFinOcVolume: {   // the candidate was formed at lam = (lo + hi) / 2
       const double V = s[0];
-      const double lam = sc->lam;
+      const double lam = 0.5 * (sc->lo + sc->hi);
+      sc->lam = lam;
       sc->V = V;
       const double err = fabs(V - sc->target);
       if (err < sc->best_err) {

@@ -3,8 +3,7 @@
 Run in the development container only (the reference tree is not present on
 GPU machines):
 
-    MPLBACKEND=Agg python tests/golden/make_golden.py               (small cases)
-    MPLBACKEND=Agg python tests/golden/make_golden.py --cantilever  (60x30x20, minutes)
+    MPLBACKEND=Agg python tests/golden/make_golden.py
 
 Only ``pytopo3d.core.optimizer.top3d`` is run: the full design trajectory
 (per-iteration compliance, the physical density snapshots, stage summaries and
@@ -56,42 +55,11 @@ def _cases():
         projection_eta=0.4, save_history=True, history_frequency=1)
 
 
-def _cantilever_case():
-    """The 60x30x20 CPU-path cantilever: reference default BC (x = 0 face fixed,
-    unit line load), volfrac 0.3, penal 3, rmin 3, beta 1, 2, 4, 8 with 25
-    iterations per stage, x0 = 0.3.  Kept small on disk: per-iteration
-    compliance and mean physical density, the stage-end densities in float32
-    and the final densities."""
-    return "cantilever_60x30x20", dict(
-        nelx=60, nely=30, nelz=20, volfrac=0.3, penal=3.0, rmin=3.0, disp_thres=0.5,
-        elem_size=1.0, tolx=1e-12, maxloop=25, beta_schedule=(1.0, 2.0, 4.0, 8.0),
-        projection_eta=0.5, move=0.2, save_history=True, history_frequency=1)
-
-
 def main():
     sys.path.insert(0, REFERENCE)
     from pytopo3d.core.optimizer import top3d  # noqa: E402  (reference, this container only)
 
     index = {}
-    if "--cantilever" in sys.argv:
-        path = os.path.join(HERE, "top3d_golden_cantilever.json")
-        name, kw = _cantilever_case()
-        diag = {}
-        rho, hist, c, _ = top3d(diagnostics_out=diag, **kw)
-        dens = np.stack(hist["density_history"])
-        stage_end = np.cumsum([25, 25, 25, 25]) - 1
-        np.savez_compressed(
-            os.path.join(HERE, f"top3d_{name}.npz"), rho=np.asarray(rho),
-            compliance_history=np.asarray(hist["compliance_history"]),
-            mean_density_history=dens.reshape(len(dens), -1).mean(axis=1),
-            stage_end_density=dens[stage_end[stage_end < len(dens)]].astype(np.float32),
-            beta_history=np.asarray(hist["beta_history"]))
-        inputs = {k: (list(v) if isinstance(v, tuple) else v) for k, v in kw.items()}
-        with open(path, "w") as fh:
-            json.dump({name: {"inputs": inputs, "final_compliance": float(c),
-                              "diagnostics": diag}}, fh, indent=1, sort_keys=True)
-        print(name, "c =", c, "iters =", diag["projection_total_iterations"])
-        return
     for name, kw in _cases():
         diag = {}
         rho, hist, c, _ = top3d(diagnostics_out=diag, **kw)

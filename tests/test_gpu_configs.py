@@ -19,12 +19,21 @@ tests could miss only at scale:
 * 320x160x160 combined form, shortened to 2 continuation stages of 3
   iterations: the mixed-precision top3d and the FP64 top3d give the same
   designs and final compliance to 1e-6.
+* 60x30x20 cantilever (the reference's CPU-path case, its default boundary
+  conditions, beta 1, 2, 4, 8 with 25 iterations each): the first design
+  iteration against the oracle's (a Jacobi-preconditioned conjugate-gradient
+  solve of the assembled matrix to 1e-12 replaces SuperLU, which needs minutes
+  per solve at this size), and the whole 100-iteration trajectory of the mixed
+  path against the FP64 path.
 """
 
 import numpy as np
 import pytest
+import scipy.sparse.linalg as spla
 import torch
 from scipy import ndimage
+
+import oracle
 
 from gpu_helpers import dev, elem_vec, note, to_dev
 from pytopo3d_amd import SIMPProblem, edge_load_force_vector, top3d
@@ -189,3 +198,39 @@ def test_mixed_matches_fp64_320x160x160():
     note("  c mixed", c_m, "fp64", c_d, "device memory in use (GB)", (total0 - free1) / 1e9)
     assert c_m == pytest.approx(c_d, rel=1e-6)
     assert np.abs(rho_m - rho_d).max() <= 1e-6
+
+
+def _jacobi_cg(Kff, bf):
+    d = 1.0 / Kff.diagonal()
+    M = spla.LinearOperator(Kff.shape, matvec=lambda v: d * v)
+    u, info = spla.cg(Kff, bf, rtol=1e-12, maxiter=50000, M=M)
+    assert info == 0
+    return u
+
+
+def test_cantilever_trajectory_60x30x20():
+    nelx, nely, nelz = 60, 30, 20
+    note("cantilever trajectory", nelx, nely, nelz)
+    kw = dict(nelx=nelx, nely=nely, nelz=nelz, volfrac=0.3, penal=3.0, rmin=3.0, disp_thres=0.5,
+              elem_size=1.0, tolx=1e-12, beta_schedule=(1.0, 2.0, 4.0, 8.0), projection_eta=0.5,
+              move=0.2, save_history=True, history_frequency=1)
+    # first design iteration: HIP FP64 vs the oracle
+    one = dict(kw, maxloop=1, beta_schedule=(1.0,))
+    rho_o, h_o, _, _ = oracle.top3d(solver=_jacobi_cg, skip_final_eval=True, **one)
+    rho_g, h_g, _, _ = top3d(precision="fp64", solver_rtol=1e-12, **one)
+    # (the HIP history also holds the final fixed-geometry evaluation, skipped here by the oracle)
+    np.testing.assert_allclose(h_g["compliance_history"][0], h_o["compliance_history"][0], rtol=1e-9)
+    np.testing.assert_allclose(h_g["density_history"][0], h_o["density_history"][0], atol=1e-9)
+    # full trajectory: mixed vs FP64
+    full = dict(kw, maxloop=25)
+    rho_d, h_d, c_d, _ = top3d(precision="fp64", solver_rtol=1e-10, **full)
+    rho_m, h_m, c_m, _ = top3d(precision="mixed", solver_rtol=1e-10, **full)
+    cd, cm = np.asarray(h_d["compliance_history"]), np.asarray(h_m["compliance_history"])
+    note("  history entries", cd.size, "final c", c_d, c_m)
+    assert cd.size == cm.size >= 100
+    assert np.abs(cm - cd).max() <= 1e-6 * np.abs(cd).max()
+    vd = np.asarray([d.mean() for d in h_d["density_history"]])
+    vm = np.asarray([d.mean() for d in h_m["density_history"]])
+    assert np.abs(vm - vd).max() <= 1e-6
+    assert np.abs(rho_m - rho_d).max() <= 1e-4
+    assert c_m == pytest.approx(c_d, rel=1e-6)
