@@ -183,13 +183,12 @@ struct PlaneRegs {
 };
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
-                                                  const T* __restrict__ Kg,
-                                                  const T* __restrict__ s,
-                                                  const uint8_t* __restrict__ fixed,
-                                                  const T* __restrict__ u, Epilogue<T> ep,
-                                                  int nty, int zc, int with_dot,
-                                                  const int* __restrict__ done, double* partials) {
+__device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& Kv,
+                                         const T* __restrict__ Kg, const T* __restrict__ s,
+                                         const uint8_t* __restrict__ fixed,
+                                         const T* __restrict__ u, const Epilogue<T>& ep, int nty,
+                                         int zc, int with_dot, const int* __restrict__ done,
+                                         double* partials) {
   if (done && *done) return;
   __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked operand)
   __shared__ T yo[3][3][kTile * kTile];   // exchanged corner contributions (f = 1, 2, 3)
@@ -438,6 +437,25 @@ __global__ __launch_bounds__(256) void k_apply_el(Grid g, SparseKE<float> Kv,
   }
 }
 
+// Two entry points with one body: the operator-only sweeps (FP32 and FP64 K u,
+// FP64 K p with the direction update) are compiled for 3 waves per SIMD, the
+// epilogue-heavy FP64 smoothing sweeps keep the compiler's register budget
+// (3 waves would spill them).
+#define TK_EL_ARGS                                                                              \
+  Grid g, SparseKE<float> Kv, const T* __restrict__ Kg, const T* __restrict__ s,                \
+      const uint8_t* __restrict__ fixed, const T* __restrict__ u, Epilogue<T> ep, int nty, int zc, \
+      int with_dot, const int* __restrict__ done, double* partials
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_apply_el(TK_EL_ARGS) {
+  apply_el<T, MODE>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials);
+}
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_apply_el_w3(
+    TK_EL_ARGS) {
+  apply_el<T, MODE>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials);
+}
+#undef TK_EL_ARGS
+
 // ---------------------------------------------------------------------------
 // Host: sector decomposition of KE.  Returns false when KE is not
 // reflection-symmetric with the orthotropic zero pattern (then the dense node
@@ -547,8 +565,14 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
   }
   SparseKE<float> kv;
   for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
-  hipLaunchKernelGGL((k_apply_el<T, MODE>), grid, dim3(256), 0, s, L.g, kv, sectors_of<T>(P), sl,
-                     L.fixed, u, ep, nty, zc, with_dot, done, P.partials);
+  constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
+  if (w3)
+    hipLaunchKernelGGL((k_apply_el_w3<T, w3 ? MODE : kModeApply>), grid, dim3(256), 0, s, L.g, kv,
+                       sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done, P.partials);
+  else
+    hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE>), grid, dim3(256), 0, s, L.g,
+                       kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
+                       P.partials);
   if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot)
     finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
 }
