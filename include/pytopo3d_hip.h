@@ -61,6 +61,9 @@ typedef struct tk_options {
   int no_graphs;          /* 0: each chunk of check_every Krylov iterations is captured once
                              per plan as a HIP graph and replayed (default); 1: kernels are
                              launched one by one */
+  int filter_kind;        /* density filter kernel: 0 LDS tiles when the stencil radius is
+                             <= 3 (default), 1 stencil gather (the only form for larger
+                             radii) */
 } tk_options;
 
 /* Fill *opt with the defaults. */

@@ -42,6 +42,7 @@ class TkOptions(ctypes.Structure):
         ("check_every", ctypes.c_int),
         ("operator_kind", ctypes.c_int),
         ("no_graphs", ctypes.c_int),
+        ("filter_kind", ctypes.c_int),
     ]
 
 

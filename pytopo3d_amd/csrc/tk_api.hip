@@ -140,6 +140,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->coarse_max_dof > 0)
       P->opt.coarse_max_dof = o->coarse_max_dof < kCoarseMaxDof ? o->coarse_max_dof : kCoarseMaxDof;
     if (o->no_graphs == 0 || o->no_graphs == 1) P->opt.no_graphs = o->no_graphs;
+    if (o->filter_kind == 0 || o->filter_kind == 1) P->opt.filter_kind = o->filter_kind;
     if (o->check_every > 0) P->opt.check_every = o->check_every;
     if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
   }
@@ -262,7 +263,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     free_plan(P);
     return 2;
   }
-  if (!filter_tiled_setup(*P, off, w)) P->ftiled = false;
+  if (P->opt.filter_kind == 1 || !filter_tiled_setup(*P, off, w)) P->ftiled = false;
   launch_filter_hs(*P, s);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -336,6 +337,7 @@ void tk_options_default(tk_options* o) {
   o->check_every = d.check_every;
   o->operator_kind = d.operator_kind;
   o->no_graphs = d.no_graphs;
+  o->filter_kind = d.filter_kind;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

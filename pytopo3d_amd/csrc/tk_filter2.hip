@@ -54,13 +54,8 @@ __global__ __launch_bounds__(256) void k_filter_tiled(
     Grid g, const FilterCols* __restrict__ colsg, const double* __restrict__ src,
     const double* __restrict__ Hs, const uint8_t* __restrict__ fl, double beta, double eta,
     double tb_eta, double den, double* __restrict__ o0, double* __restrict__ o1,
-    double* __restrict__ o2, const double* __restrict__ ratio, double move, double* partials,
-    Scalars* sc) {
+    double* __restrict__ o2, double* partials, Scalars* sc) {
   if (MODE == kFOcVolume && sc->oc_done) return;
-  // kFOcVolume: the staged values are the OC candidate at the bisection
-  // midpoint, formed from the design (src) and the normalised ratio
-  double lam = 0.0;
-  if (MODE == kFOcVolume) lam = 0.5 * (sc->lo + sc->hi);
   extern __shared__ double tile[];
   __shared__ FilterCols cols;
   {
@@ -77,18 +72,23 @@ __global__ __launch_bounds__(256) void k_filter_tiled(
   const int y0 = (rr % gridDim.x) * FTY, x0 = (rr / gridDim.x) * FTX, z0 = bz * FTZ;
   const int exy = g.nelx * g.nely;
   constexpr int ntile = zs * TZH;
-  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+  // all global loads of the tile are issued before the first LDS store, so
+  // their latencies overlap instead of adding up
+  constexpr int NT = (ntile + 255) / 256;
+  double stg[NT];
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int t = threadIdx.x + 256 * k;
     const int yy = t % TYH, r = t / TYH, xx = r % TXH, zz = r / TXH;
     const int ey = y0 - H + yy, ex = x0 - H + xx, ez = z0 - H + zz;
-    double v = 0.0;
-    if ((unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx &&
-        (unsigned)ez < (unsigned)g.nelz) {
-      const int e = ey + ex * g.nely + ez * exy;
-      if (MODE == kFOcVolume) v = oc_candidate(fl[e], src[e], ratio[e], lam, move);
-      else v = src[e];
-    }
-    tile[t] = v;
+    stg[k] = (t < ntile && (unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx &&
+              (unsigned)ez < (unsigned)g.nelz)
+                 ? src[ey + ex * g.nely + ez * exy]
+                 : 0.0;
   }
+#pragma unroll
+  for (int k = 0; k < NT; ++k)
+    if (threadIdx.x + 256 * k < ntile) tile[threadIdx.x + 256 * k] = stg[k];
   __syncthreads();
   const int ty = threadIdx.x % FTY, tx = threadIdx.x / FTY;
   double acc[FTZ];
@@ -183,15 +183,19 @@ static dim3 fgrid(const Grid& g) {
   return dim3((g.nely + FTY - 1) / FTY, (g.nelx + FTX - 1) / FTX, (g.nelz + FTZ - 1) / FTZ);
 }
 
+int filter_blocks(const Plan& P) {
+  const dim3 gg = fgrid(P.g);
+  return (int)(gg.x * gg.y * gg.z);
+}
+
 template <int MODE>
 static void launch_tiled(Plan& P, const double* src, double beta, double eta, double tb,
-                         double den, double* o0, double* o1, double* o2, hipStream_t s,
-                         const double* ratio = nullptr, double move = 0.0) {
-  const dim3 gg = fgrid(P.g);
+                         double den, double* o0, double* o1, double* o2, hipStream_t s) {
   const FilterCols* fc = (const FilterCols*)P.fcols;
+  const dim3 gg = fgrid(P.g);
 #define TK_LAUNCH_H(HH)                                                                      \
   hipLaunchKernelGGL((k_filter_tiled<MODE, HH>), gg, dim3(256), P.flds, s, P.g, fc, src, P.Hs, \
-                     P.eflags, beta, eta, tb, den, o0, o1, o2, ratio, move, P.partials, P.sc)
+                     P.eflags, beta, eta, tb, den, o0, o1, o2, P.partials, P.sc)
   switch (P.fhalo) {
     case 0: TK_LAUNCH_H(0); break;
     case 1: TK_LAUNCH_H(1); break;
@@ -213,17 +217,15 @@ void tiled_filter_plain(Plan& P, const double* v, double* out, hipStream_t s) {
 void tiled_free_sum(Plan& P, const double* x, double beta, double eta, double tb, double den,
                     int slot, hipStream_t s) {
   launch_tiled<kFSum>(P, x, beta, eta, tb, den, nullptr, nullptr, nullptr, s);
-  const dim3 gg = fgrid(P.g);
-  finalize(P, slot == 0 ? kFinSum0 : kFinSum1, (int)(gg.x * gg.y * gg.z), s);
+  finalize(P, slot == 0 ? kFinSum0 : kFinSum1, filter_blocks(P), s);
 }
 
-// One bisection step: candidate at the midpoint (formed while staging the
-// tile), filtered, projected and summed over free elements, then kFinOcVolume.
-void tiled_oc_volume(Plan& P, const double* x, const double* ratio, double move, double beta,
-                     double eta, double tb, double den, hipStream_t s) {
-  launch_tiled<kFOcVolume>(P, x, beta, eta, tb, den, nullptr, nullptr, nullptr, s, ratio, move);
-  const dim3 gg = fgrid(P.g);
-  finalize(P, kFinOcVolume, (int)(gg.x * gg.y * gg.z), s);
+// Volume of one bisection candidate: filtered, projected and summed over free
+// elements, then kFinOcVolume.
+void tiled_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb, double den,
+                     hipStream_t s) {
+  launch_tiled<kFOcVolume>(P, cand, beta, eta, tb, den, nullptr, nullptr, nullptr, s);
+  finalize(P, kFinOcVolume, filter_blocks(P), s);
 }
 
 }  // namespace tk

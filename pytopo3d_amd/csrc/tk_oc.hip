@@ -144,17 +144,18 @@ void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double m
                      P.eflags, x, ratio, move, use_best, cand, P.sc);
 }
 
-// One bisection step at lambda = (lo + hi) / 2: the candidate (formed inside
-// the tiled filter, or by k_oc_candidate into cand for the gather filter), its
+// One bisection step at lambda = (lo + hi) / 2: the candidate (k_oc_candidate;
+// forming it inside the tiled filter instead measured slower, the square root
+// and division being repeated on the 2.5x larger halo-padded tile), its
 // free-element projected volume, and the bracket update (kFinOcVolume).
 void launch_oc_step(Plan& P, const double* x, const double* ratio, double move, double* cand,
                     double beta, double eta, double tb_eta, double den, hipStream_t s) {
   P.cnt.oc_steps++;
+  launch_oc_candidate(P, x, ratio, move, 0, cand, s);
   if (P.ftiled) {
-    tiled_oc_volume(P, x, ratio, move, beta, eta, tb_eta, den, s);
+    tiled_oc_volume(P, cand, beta, eta, tb_eta, den, s);
     return;
   }
-  launch_oc_candidate(P, x, ratio, move, 0, cand, s);
   hipLaunchKernelGGL(k_oc_volume, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.nsten,
                      P.sten_off, P.sten_w, P.Hs, P.eflags, cand, beta, eta, tb_eta, den,
                      P.partials, P.counter, P.sc);

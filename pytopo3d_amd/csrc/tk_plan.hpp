@@ -37,6 +37,7 @@ struct Options {
   int check_every = 8;           // Krylov convergence is read back every N iterations
   int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node kernel
   int no_graphs = 0;             // 1: no captured solver chunks
+  int filter_kind = 0;           // 0: LDS tiles when the radius allows, 1: stencil gather
 };
 
 struct Counters {
@@ -169,8 +170,8 @@ void tiled_filter_project(Plan& P, const double* x, double beta, double eta, dou
 void tiled_filter_plain(Plan& P, const double* v, double* out, hipStream_t s);
 void tiled_free_sum(Plan& P, const double* x, double beta, double eta, double tb, double den,
                     int slot, hipStream_t s);
-void tiled_oc_volume(Plan& P, const double* x, const double* ratio, double move, double beta,
-                     double eta, double tb, double den, hipStream_t s);
+void tiled_oc_volume(Plan& P, const double* cand, double beta, double eta, double tb, double den,
+                     hipStream_t s);
 template <typename T>
 void el_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, int with_dot,
               const int* done, hipStream_t s);

@@ -16,7 +16,8 @@ from gpu_helpers import dev, elem_vec, make_plan, note, random_masks, to_dev
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(3, 2, 4, 1.6), (6, 4, 3, 1.5), (7, 5, 4, 2.5), (9, 6, 5, 3.0), (12, 5, 7, 2.0)]
+GRIDS = [(3, 2, 4, 1.6), (6, 4, 3, 1.5), (7, 5, 4, 2.5), (9, 6, 5, 3.0), (12, 5, 7, 2.0),
+         (37, 66, 21, 2.4)]
 
 
 def _stencil_matrix(P, nelx, nely, nelz):
@@ -49,14 +50,18 @@ def test_stencil_and_row_sums_match_oracle(nelx, nely, nelz, rmin):
     P.close()
 
 
+FILTER_KINDS = [0, 1]   # LDS tiles, stencil gather
+
+
+@pytest.mark.parametrize("fk", FILTER_KINDS)
 @pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS)
 @pytest.mark.parametrize("beta", [1.0, 8.0])
-def test_filter_project_matches_oracle(nelx, nely, nelz, rmin, beta):
-    note("filter_project", nelx, nely, nelz, rmin, beta)
+def test_filter_project_matches_oracle(nelx, nely, nelz, rmin, beta, fk):
+    note("filter_project", nelx, nely, nelz, rmin, beta, "filter_kind", fk)
     rng = np.random.default_rng(nelx * 100 + nely)
     shape = (nely, nelx, nelz)
     solid, void = random_masks(rng, shape)
-    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void)
+    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void, options={"filter_kind": fk})
     x = rng.uniform(0.0, 1.0, shape)
     x[solid] = 1.0
     x[void] = 0.0
@@ -75,13 +80,14 @@ def test_filter_project_matches_oracle(nelx, nely, nelz, rmin, beta):
     P.close()
 
 
+@pytest.mark.parametrize("fk", FILTER_KINDS)
 @pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS)
-def test_filter_transpose_matches_oracle_chain_rule(nelx, nely, nelz, rmin):
-    note("transpose2", nelx, nely, nelz, rmin)
+def test_filter_transpose_matches_oracle_chain_rule(nelx, nely, nelz, rmin, fk):
+    note("transpose2", nelx, nely, nelz, rmin, "filter_kind", fk)
     rng = np.random.default_rng(7 + nelz)
     shape = (nely, nelx, nelz)
     solid, void = random_masks(rng, shape)
-    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void)
+    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void, options={"filter_kind": fk})
     H, Hs = oracle.filter_matrix(nelx, nely, nelz, rmin)
     g = rng.standard_normal(shape)
     dr = rng.uniform(0, 2, shape)
@@ -197,16 +203,17 @@ def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin):
     P.close()
 
 
+@pytest.mark.parametrize("fk", FILTER_KINDS)
 @pytest.mark.parametrize("scale", [1.0, 1e-18, 1e18])
 @pytest.mark.parametrize("nelx,nely,nelz,rmin,beta", [(6, 4, 3, 1.5, 1.0), (10, 6, 4, 2.5, 4.0),
-                                                      (12, 8, 6, 3.0, 8.0)])
-def test_oc_update_matches_oracle(nelx, nely, nelz, rmin, beta, scale):
-    note("oc", nelx, nely, nelz, rmin, beta, scale)
+                                                      (12, 8, 6, 3.0, 8.0), (40, 20, 35, 3.0, 2.0)])
+def test_oc_update_matches_oracle(nelx, nely, nelz, rmin, beta, scale, fk):
+    note("oc", nelx, nely, nelz, rmin, beta, scale, "filter_kind", fk)
     rng = np.random.default_rng(17)
     shape = (nely, nelx, nelz)
     solid, void = random_masks(rng, shape, 0.05, 0.05)
     free = ~(solid | void)
-    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void)
+    P, _ = make_plan(nelx, nely, nelz, rmin, solid, void, options={"filter_kind": fk})
     H, Hs = oracle.filter_matrix(nelx, nely, nelz, rmin)
     x = rng.uniform(0.1, 0.9, shape)
     x[solid], x[void] = 1.0, 0.0
