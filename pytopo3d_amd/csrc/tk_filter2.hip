@@ -1,7 +1,7 @@
 // LDS-tiled density filter (fast path of H x, used for the forward filter,
 // its transpose and every OC bisection volume).
 //
-// A workgroup owns a 32 (y) x 8 (x) x 8 (z) tile of outputs and stages the
+// A workgroup owns a 32 (y) x 16 (x) x 8 (z) tile of outputs and stages the
 // tile plus an H-element halo (H = largest stencil offset, <= 3) in LDS, with
 // zero padding outside the domain -- padding with zeros is exactly the
 // boundary truncation of pytopo3d/utils/filter.py:122-228.  Each thread owns
@@ -21,7 +21,8 @@
 
 namespace tk {
 
-constexpr int FTY = 32, FTX = 8, FTZ = 8, FHMAX = 3;
+constexpr int FTY = 32, FTX = 16, FTZ = 8, FHMAX = 3;
+constexpr int FTHREADS = FTY * FTX;   // one thread per (y, x) output column
 constexpr int FMAXCOL = 64;       // (2*3+1)^2 = 49 columns at most
 constexpr int FMAXW = 343;        // weights of all columns, (2*3+1)^3 at most
 
@@ -50,7 +51,7 @@ __device__ __forceinline__ void col_accum(const double* __restrict__ tile, int b
 }
 
 template <int MODE, int H>
-__global__ __launch_bounds__(256) void k_filter_tiled(
+__global__ __launch_bounds__(FTHREADS) void k_filter_tiled(
     Grid g, const FilterCols* __restrict__ colsg, const double* __restrict__ src,
     const double* __restrict__ Hs, const uint8_t* __restrict__ fl, double beta, double eta,
     double tb_eta, double den, double* __restrict__ o0, double* __restrict__ o1,
@@ -74,11 +75,11 @@ __global__ __launch_bounds__(256) void k_filter_tiled(
   constexpr int ntile = zs * TZH;
   // all global loads of the tile are issued before the first LDS store, so
   // their latencies overlap instead of adding up
-  constexpr int NT = (ntile + 255) / 256;
+  constexpr int NT = (ntile + FTHREADS - 1) / FTHREADS;
   double stg[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
-    const int t = threadIdx.x + 256 * k;
+    const int t = threadIdx.x + FTHREADS * k;
     const int yy = t % TYH, r = t / TYH, xx = r % TXH, zz = r / TXH;
     const int ey = y0 - H + yy, ex = x0 - H + xx, ez = z0 - H + zz;
     stg[k] = (t < ntile && (unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx &&
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(256) void k_filter_tiled(
   }
 #pragma unroll
   for (int k = 0; k < NT; ++k)
-    if (threadIdx.x + 256 * k < ntile) tile[threadIdx.x + 256 * k] = stg[k];
+    if (threadIdx.x + FTHREADS * k < ntile) tile[threadIdx.x + FTHREADS * k] = stg[k];
   __syncthreads();
   const int ty = threadIdx.x % FTY, tx = threadIdx.x / FTY;
   double acc[FTZ];
@@ -172,6 +173,25 @@ bool filter_tiled_setup(Plan& P, const std::vector<int>& off, const std::vector<
   if (hipMemcpy(P.fcols, &fc, sizeof(FilterCols), hipMemcpyHostToDevice) != hipSuccess) return false;
   P.fhalo = H;
   const size_t lds = sizeof(double) * (FTY + 2 * H) * (FTX + 2 * H) * (FTZ + 2 * H);
+  // tiles above 64 KiB need the kernels' dynamic LDS limit raised
+  auto allow = [&](const void* f) {
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+           hipSuccess;
+  };
+  bool ok = true;
+#define TK_ALLOW(HH)                                                                   \
+  ok = ok && allow((const void*)k_filter_tiled<kFProject, HH>) &&                      \
+       allow((const void*)k_filter_tiled<kFSum, HH>) &&                                \
+       allow((const void*)k_filter_tiled<kFOcVolume, HH>) &&                           \
+       allow((const void*)k_filter_tiled<kFPlain, HH>)
+  switch (H) {
+    case 0: TK_ALLOW(0); break;
+    case 1: TK_ALLOW(1); break;
+    case 2: TK_ALLOW(2); break;
+    default: TK_ALLOW(3); break;
+  }
+#undef TK_ALLOW
+  if (!ok) return false;
   P.flds = lds;
   P.ftiled = true;
   return true;
@@ -194,7 +214,8 @@ static void launch_tiled(Plan& P, const double* src, double beta, double eta, do
   const FilterCols* fc = (const FilterCols*)P.fcols;
   const dim3 gg = fgrid(P.g);
 #define TK_LAUNCH_H(HH)                                                                      \
-  hipLaunchKernelGGL((k_filter_tiled<MODE, HH>), gg, dim3(256), P.flds, s, P.g, fc, src, P.Hs, \
+  hipLaunchKernelGGL((k_filter_tiled<MODE, HH>), gg, dim3(FTHREADS), P.flds, s, P.g, fc, src,    \
+                     P.Hs, \
                      P.eflags, beta, eta, tb, den, o0, o1, o2, P.partials, P.sc)
   switch (P.fhalo) {
     case 0: TK_LAUNCH_H(0); break;
