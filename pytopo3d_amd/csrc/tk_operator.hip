@@ -270,14 +270,9 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_fixed(Grid f, Grid c,
 // neighbourhood with whole-node (3-component) loads; neighbouring threads
 // share fine nodes through L1.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
-                                                      const uint8_t* __restrict__ fc,
-                                                      const T* __restrict__ rf,
-                                                      T* __restrict__ bc,
-                                                      const int* __restrict__ done) {
-  if (done && *done) return;
-  const int I = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (I >= c.nnode) return;
+__device__ __forceinline__ void restrict_node(const Grid& f, const Grid& c,
+                                              const uint8_t* __restrict__ fc, const T* rf, T* bc,
+                                              int I) {
   int Iy, Ix, Iz;
   node_coords(c, I, Iy, Ix, Iz);
   T a0 = T(0), a1 = T(0), a2 = T(0);
@@ -292,13 +287,13 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
       const int ix = 2 * Ix + dx;
       if ((unsigned)ix >= (unsigned)f.nnx) continue;
       const T wzx = wz * (dx ? T(0.5) : T(1));
-      const T* __restrict__ row = rf + 3 * ((size_t)ix * f.nny + (size_t)iz * nxy);
+      const T* row = rf + 3 * ((size_t)ix * f.nny + (size_t)iz * nxy);
 #pragma unroll
       for (int dy = -1; dy <= 1; ++dy) {
         const int iy = 2 * Iy + dy;
         if ((unsigned)iy >= (unsigned)f.nny) continue;
         const T wgt = wzx * (dy ? T(0.5) : T(1));
-        const T* __restrict__ q = row + 3 * iy;
+        const T* q = row + 3 * iy;
         a0 += wgt * q[0];
         a1 += wgt * q[1];
         a2 += wgt * q[2];
@@ -306,23 +301,30 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
     }
   }
   const uint8_t fI = fc[I];
-  T* __restrict__ o = bc + 3 * (size_t)I;
+  T* o = bc + 3 * (size_t)I;
   o[0] = (fI & 1) ? T(0) : a0;
   o[1] = (fI & 2) ? T(0) : a1;
   o[2] = (fI & 4) ? T(0) : a2;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
+                                                      const uint8_t* __restrict__ fc,
+                                                      const T* __restrict__ rf,
+                                                      T* __restrict__ bc,
+                                                      const int* __restrict__ done) {
+  if (done && *done) return;
+  const int I = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (I >= c.nnode) return;
+  restrict_node(f, c, fc, rf, bc, I);
+}
+
 // x_f += P x_c on free fine DOFs.  Branch-free: an even fine index has one
 // coarse parent per axis, given weight 1 and a duplicate with weight 0.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
-                                                         const uint8_t* __restrict__ ff,
-                                                         const T* __restrict__ xc,
-                                                         T* __restrict__ xf,
-                                                         const int* __restrict__ done) {
-  if (done && *done) return;
-  const int i = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= f.nnode) return;
+__device__ __forceinline__ void prolong_node(const Grid& c, const Grid& f,
+                                             const uint8_t* __restrict__ ff, const T* xc, T* xf,
+                                             int i) {
   int iy, ix, iz;
   node_coords(f, i, iy, ix, iz);
   const int y0 = iy >> 1, y1 = (iy + 1) >> 1;
@@ -338,17 +340,29 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
     const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
     const int I = (ky ? y1 : y0) + (kx ? x1 : x0) * c.nny + (kz ? z1 : z0) * cxy;
     const T w = wy[ky] * wx[kx] * wz[kz];
-    const T* __restrict__ q = xc + 3 * (size_t)I;
+    const T* q = xc + 3 * (size_t)I;
     a0 += w * q[0];
     a1 += w * q[1];
     a2 += w * q[2];
   }
   const uint8_t fi = ff[i];
-  T* __restrict__ o = xf + 3 * (size_t)i;
+  T* o = xf + 3 * (size_t)i;
   const T v0 = o[0], v1 = o[1], v2 = o[2];
   o[0] = (fi & 1) ? v0 : v0 + a0;
   o[1] = (fi & 2) ? v1 : v1 + a1;
   o[2] = (fi & 4) ? v2 : v2 + a2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
+                                                         const uint8_t* __restrict__ ff,
+                                                         const T* __restrict__ xc,
+                                                         T* __restrict__ xf,
+                                                         const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= f.nnode) return;
+  prolong_node(c, f, ff, xc, xf, i);
 }
 
 // ------------------------------------------------------------------ coarsest grid
