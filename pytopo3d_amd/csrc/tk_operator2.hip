@@ -222,19 +222,18 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   const int xny = Y0 - 1 + xi, xnx = X0 - 1 + xj;
   const bool ext_node = xi >= 0 && (unsigned)xny < (unsigned)g.nny && (unsigned)xnx < (unsigned)g.nnx;
 
-  // kModeApplyDir: direction value at DOF 3n + c (first iteration: beta = 0
-  // and p32o is not read)
-  // Without a previous direction the second operand is z32 itself with a
-  // zero coefficient (finite whenever z32 is), so the loads stay unconditional.
-  T dir_zs = T(0), dir_beta = T(0);
+  // kModeApplyDir: direction value at DOF 3n + c, formed in FP32 (the
+  // direction is stored in FP32 anyway; K p is then taken in T).  Without a
+  // previous direction the second operand is z32 itself with a zero
+  // coefficient (finite whenever z32 is), so the loads stay unconditional.
+  float dir_zs = 0.0f, dir_beta = 0.0f;
   const float* __restrict__ dir_p = ep.p32o ? ep.p32o : ep.z32;
   if constexpr (MODE == kModeApplyDir) {
-    dir_zs = (T)ep.sc->zs;
-    dir_beta = ep.p32o ? (T)ep.sc->beta : T(0);
+    dir_zs = (float)ep.sc->zs;
+    dir_beta = ep.p32o ? (float)ep.sc->beta : 0.0f;
   }
   auto dir_value = [&](int n, int c) -> T {
-    const T z = (T)ep.z32[3 * n + c], po = (T)dir_p[3 * n + c];
-    return (T)(float)(dir_zs * z + dir_beta * po);
+    return (T)(dir_zs * ep.z32[3 * n + c] + dir_beta * dir_p[3 * n + c]);
   };
   // global loads of node plane pz (registers only)
   auto fetch = [&](int pz, PlaneRegs<T, MODE>& R) {
