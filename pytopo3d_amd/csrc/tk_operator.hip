@@ -275,31 +275,40 @@ __device__ __forceinline__ void restrict_node(const Grid& f, const Grid& c,
                                               int I) {
   int Iy, Ix, Iz;
   node_coords(c, I, Iy, Ix, Iz);
+  // fine neighbours 2I - 1, 2I, 2I + 1 per axis; indices outside the fine grid
+  // are clamped and weighted 0, so all 27 loads are unconditional
+  int jy[3], jx[3], jz[3];
+  T wy[3], wx[3], wz[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T w = (k == 1) ? T(1) : T(0.5);
+    int t = 2 * Iy + k - 1;
+    wy[k] = ((unsigned)t < (unsigned)f.nny) ? w : T(0);
+    jy[k] = min(max(t, 0), f.nny - 1);
+    t = 2 * Ix + k - 1;
+    wx[k] = ((unsigned)t < (unsigned)f.nnx) ? w : T(0);
+    jx[k] = min(max(t, 0), f.nnx - 1);
+    t = 2 * Iz + k - 1;
+    wz[k] = ((unsigned)t < (unsigned)f.nnz) ? w : T(0);
+    jz[k] = min(max(t, 0), f.nnz - 1);
+  }
   T a0 = T(0), a1 = T(0), a2 = T(0);
   const int nxy = f.nnx * f.nny;
 #pragma unroll
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int iz = 2 * Iz + dz;
-    if ((unsigned)iz >= (unsigned)f.nnz) continue;
-    const T wz = dz ? T(0.5) : T(1);
+  for (int kz = 0; kz < 3; ++kz)
 #pragma unroll
-    for (int dx = -1; dx <= 1; ++dx) {
-      const int ix = 2 * Ix + dx;
-      if ((unsigned)ix >= (unsigned)f.nnx) continue;
-      const T wzx = wz * (dx ? T(0.5) : T(1));
-      const T* row = rf + 3 * ((size_t)ix * f.nny + (size_t)iz * nxy);
+    for (int kx = 0; kx < 3; ++kx) {
+      const T wzx = wz[kz] * wx[kx];
+      const T* row = rf + 3 * ((size_t)jx[kx] * f.nny + (size_t)jz[kz] * nxy);
 #pragma unroll
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int iy = 2 * Iy + dy;
-        if ((unsigned)iy >= (unsigned)f.nny) continue;
-        const T wgt = wzx * (dy ? T(0.5) : T(1));
-        const T* q = row + 3 * iy;
+      for (int ky = 0; ky < 3; ++ky) {
+        const T wgt = wzx * wy[ky];
+        const T* q = row + 3 * jy[ky];
         a0 += wgt * q[0];
         a1 += wgt * q[1];
         a2 += wgt * q[2];
       }
     }
-  }
   const uint8_t fI = fc[I];
   T* o = bc + 3 * (size_t)I;
   o[0] = (fI & 1) ? T(0) : a0;

@@ -57,6 +57,9 @@ struct Epilogue {
                    // element scales (KE's diagonal is the same at every corner)
   T c0, c1, c2;
   T kd[3];         // KE diagonal per component
+  // MODE 6: coarse-grid correction, prolongated on the fly
+  const T* xc;
+  Grid cg;
   // MODE 5 (flexible CG direction): p = zs z32 + beta p32o, stored as p32n
   const float* z32;
   const float* p32o;
@@ -77,8 +80,12 @@ enum {
   kModeChebFirst = 2,
   kModeChebStep = 3,
   kModeChebZero = 4,
-  kModeApplyDir = 5
+  kModeApplyDir = 5,
+  kModeChebFirstP = 6
 };
+// kModeChebFirstP is kModeChebFirst on x + P xc (the coarse-grid correction
+// prolongated on the fly while fetching the node values; it is added on free
+// DOFs only), so the post-smoothing needs no separate prolongation sweep.
 
 constexpr int kTile = 16;          // threads per tile edge
 constexpr int kOut = kTile - 1;    // owned nodes per tile edge
@@ -194,7 +201,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   __shared__ T yo[3][3][kTile * kTile];   // exchanged corner contributions (f = 1, 2, 3)
   __shared__ T kb[kSecN];                 // FP64 sector entries
   __shared__ T ss[kTile * kTile];         // element scales of the layer (on-the-fly diagonal)
-  constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep;
+  constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep || MODE == kModeChebFirstP;
   const bool diag_fly = kCheb && ep.dinv == nullptr;
 
   const int tid = threadIdx.x;
@@ -235,6 +242,27 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   auto dir_value = [&](int n, int c) -> T {
     return (T)(dir_zs * ep.z32[3 * n + c] + dir_beta * dir_p[3 * n + c]);
   };
+  // kModeChebFirstP: (P xc) at fine node (iy, ix, iz), trilinear weights (an
+  // even fine index has one coarse parent per axis: weight 1, duplicate 0)
+  auto prol = [&](int iy, int ix, int iz, T out[3]) {
+    const Grid& c = ep.cg;
+    const int y0 = iy >> 1, y1 = (iy + 1) >> 1, x0 = ix >> 1, x1 = (ix + 1) >> 1;
+    const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
+    const T wy[2] = {(iy & 1) ? T(0.5) : T(1), (iy & 1) ? T(0.5) : T(0)};
+    const T wx[2] = {(ix & 1) ? T(0.5) : T(1), (ix & 1) ? T(0.5) : T(0)};
+    const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
+    const int cxy = c.nnx * c.nny;
+    out[0] = out[1] = out[2] = T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+      const int I = (ky ? y1 : y0) + (kx ? x1 : x0) * c.nny + (kz ? z1 : z0) * cxy;
+      const T w = wy[ky] * wx[kx] * wz[kz];
+      out[0] += w * ep.xc[3 * I];
+      out[1] += w * ep.xc[3 * I + 1];
+      out[2] += w * ep.xc[3 * I + 2];
+    }
+  };
   // global loads of node plane pz (registers only)
   auto fetch = [&](int pz, PlaneRegs<T, MODE>& R) {
 #pragma unroll
@@ -243,6 +271,8 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     if (col_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = ey + ex * g.nny + pz * nxy;
       R.fix = fixed[n];
+      T pc[3] = {T(0), T(0), T(0)};
+      if constexpr (MODE == kModeChebFirstP) prol(ey, ex, pz, pc);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if constexpr (MODE == kModeChebZero) {
@@ -251,6 +281,8 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
           R.own[c] = ep.c0 * R.odi[c] * R.ob[c];
         } else if constexpr (MODE == kModeApplyDir) {
           R.own[c] = dir_value(n, c);
+        } else if constexpr (MODE == kModeChebFirstP) {
+          R.own[c] = u[3 * n + c] + (((R.fix >> c) & 1) ? T(0) : pc[c]);
         } else {
           R.own[c] = u[3 * n + c];
         }
@@ -260,11 +292,14 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     if (ext_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = xny + xnx * g.nny + pz * nxy;
       const uint8_t f = fixed[n];
+      T pc[3] = {T(0), T(0), T(0)};
+      if constexpr (MODE == kModeChebFirstP) prol(xny, xnx, pz, pc);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         T v;
         if constexpr (MODE == kModeChebZero) v = ep.c0 * ep.dinv[3 * n + c] * ep.b[3 * n + c];
         else if constexpr (MODE == kModeApplyDir) v = dir_value(n, c);
+        else if constexpr (MODE == kModeChebFirstP) v = u[3 * n + c] + pc[c];
         else v = u[3 * n + c];
         R.ext[c] = ((f >> c) & 1) ? T(0) : v;
       }
@@ -399,7 +434,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         } else if (MODE == kModeResidual) {
           // zero on fixed DOFs: the restriction reads no fine mask
           ep.y[i] = fx ? T(0) : eb[c] - val;
-        } else if (MODE == kModeChebFirst) {
+        } else if (MODE == kModeChebFirst || MODE == kModeChebFirstP) {
           const T dd = ep.c0 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
           ep.xo[i] = raw_prev[c] + dd;
@@ -627,6 +662,24 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   else launch_el<T, kModeChebStep>(P, L, sl, x, ep, 0, done, s);
 }
 
+// First post-smoothing step on x + P xc (see kModeChebFirstP).
+template <typename T>
+void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const T* sl,
+                     const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,
+                     const int* done, hipStream_t s) {
+  Epilogue<T> ep{};
+  ep.b = b;
+  ep.xo = xo;
+  ep.d = write_d ? d : nullptr;
+  ep.dp = d;
+  ep.dinv = P.kd_ok ? nullptr : dinv;
+  for (int c = 0; c < 3; ++c) ep.kd[c] = (T)P.kd[c];
+  ep.c0 = (T)c0;
+  ep.xc = xc;
+  ep.cg = C.g;
+  launch_el<T, kModeChebFirstP>(P, L, sl, x, ep, 0, done, s);
+}
+
 // First two Chebyshev steps from x = 0 in one sweep (see kModeChebZero).
 template <typename T>
 void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, T* xo, T* d,
@@ -650,7 +703,10 @@ void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T*
   template void el_cheb<T>(Plan&, const Level&, const T*, const T*, const T*, const T*, T*, T*, \
                            double, double, double, int, int, const int*, hipStream_t);  \
   template void el_cheb_zero2<T>(Plan&, const Level&, const T*, const T*, const T*, T*, T*,     \
-                                 double, double, double, int, const int*, hipStream_t);
+                                 double, double, double, int, const int*, hipStream_t);         \
+  template void el_cheb_prolong<T>(Plan&, const Level&, const Level&, const T*, const T*,       \
+                                   const T*, const T*, const T*, T*, T*, double, int,           \
+                                   const int*, hipStream_t);
 TK_INST2(double)
 TK_INST2(float)
 #undef TK_INST2

@@ -187,6 +187,10 @@ void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T*
                    double c0, double c1, double c2, int write_d, const int* done, hipStream_t s);
 void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, double* y,
                   hipStream_t s);
+template <typename T>
+void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const T* sl,
+                     const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,
+                     const int* done, hipStream_t s);
 void launch_coarse_setup(Plan& P, hipStream_t s);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);

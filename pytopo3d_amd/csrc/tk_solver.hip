@@ -214,8 +214,13 @@ static int read_scalars(Plan& P, hipStream_t s) {
 
 // Chebyshev smoothing of level l: `deg` steps from x (x_zero: from 0).
 // Returns the buffer holding the result (x or its ping-pong partner).
+constexpr int kProlongFuseMaxNodes = 100000;
+
+// xc != nullptr (post-smoothing, element kernel): the smoothing starts from
+// x + P xc, the coarse-grid correction being prolongated inside the first step.
 template <typename T>
-static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s) {
+static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s,
+                 const T* xc = nullptr) {
   Level& L = P.lv[l];
   LevelBuf<T>& B = buf<T>(L);
   const double lmax = P.lam_bound * P.opt.cheb_hi, lmin = P.lam_bound * P.opt.cheb_lo;
@@ -234,6 +239,13 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
     std::swap(x, xa);
     rho_old = rho_new;
     k = 2;
+  } else if (xc) {
+    if (&L == &P.lv[0]) {
+      if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
+    }
+    el_cheb_prolong<T>(P, L, P.lv[l + 1], xc, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta,
+                       P.opt.smooth_degree > 1, &P.sc->done, s);
+    std::swap(x, xa);
   } else {
     launch_cheb_first<T>(P, L, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta, x_zero,
                          P.opt.smooth_degree > 1, s);
@@ -274,8 +286,15 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s) {
     cur[nl - 1] = C.x;
   }
   for (int l = nl - 2; l >= 0; --l) {
-    launch_prolong_add<T>(P.lv[l + 1], P.lv[l], cur[l + 1], cur[l], s, done);
-    cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s);
+    // small levels: the prolongation rides in the first smoothing sweep (one
+    // launch less); large levels: the separate sweep is faster than gathering
+    // the coarse correction inside the register-heavy smoothing kernel
+    if (P.use_el && P.lv[l].g.nnode <= kProlongFuseMaxNodes) {
+      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s, cur[l + 1]);
+    } else {
+      launch_prolong_add<T>(P.lv[l + 1], P.lv[l], cur[l + 1], cur[l], s, done);
+      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s);
+    }
   }
   return cur[0];
 }
