@@ -611,7 +611,8 @@ int tk_oc_update(tk_plan* plan, const double* x, const double* dc_dx, const doub
   tk::launch_oc_ratio(P, dc_dx, dv_dx, P.ework0, s);
   tk::launch_oc_normalize(P, P.ework0, s);
   tk::launch_oc_init(P, target, max_steps, tol, s);
-  for (int k = 0; k < max_steps; ++k) {
+  // one more evaluation than steps: a probe that overshoots settles none
+  for (int k = 0; k <= max_steps; ++k) {
     tk::launch_oc_step(P, x, P.ework0, move, P.ework1, beta, eta, tb, den, s);
   }
   tk::launch_oc_candidate(P, x, P.ework0, move, 1, x_new, s);

@@ -53,7 +53,8 @@ struct Scalars {
   int iter, done, maxit, stalled;
   // optimality-criteria bisection
   double lo, hi, lam, best_lam, best_err, target, V, ratio_scale, tol;
-  int steps, oc_done, max_steps, pad1;
+  int steps, oc_done, max_steps, pend;   // pend: bisection steps the next evaluation resolves
+  int gallop, pad1;                      // gallop: the lower bound is still 0, probe ahead
   // general results
   double c;            // compliance
   double change;       // max |x_new - x| over free elements

@@ -8,6 +8,11 @@
 //   filtered candidate; the lambda with the smallest |V - target| (strict <)
 //   is kept; V > target raises the lower bound; stop when
 //   (hi - lo) / max(hi + lo, tiny) < tol or after max_steps.
+// While the lower bound is still 0 the bisection midpoints are hi/2, hi/4, ...
+// and V is non-increasing in lambda, so one evaluation at the k-th of them
+// that lands at or below the target settles all k steps (each sets hi = mid);
+// the probe length doubles while that holds (see kFinOcVolume), which skips
+// most of the ~40 steps from 1e12 down to the multiplier's scale.
 // The kept candidate is rebuilt from the kept lambda with the same formula
 // (oc_candidate), so it is bitwise the candidate that was measured.
 // Also: change / volume / gray statistics (pytopo3d/core/optimizer.py:642-655).
@@ -55,6 +60,9 @@ __global__ void k_oc_init(Scalars* sc, double target, int max_steps, double tol)
   sc->oc_done = 0;
   sc->max_steps = max_steps;
   sc->tol = tol;
+  sc->pend = 1;
+  sc->gallop = 1;
+  sc->lam = 0.5 * (sc->lo + sc->hi);
 }
 
 __global__ __launch_bounds__(kBlock) void k_oc_candidate(int n, const uint8_t* __restrict__ fl,
@@ -64,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_candidate(int n, const uint8_t* _
                                                           double* __restrict__ cand,
                                                           Scalars* sc) {
   if (!use_best && sc->oc_done) return;
-  const double lam = use_best ? sc->best_lam : 0.5 * (sc->lo + sc->hi);
+  const double lam = use_best ? sc->best_lam : sc->lam;   // sc->lam: set by init / finalize
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const uint8_t f = fl[e];

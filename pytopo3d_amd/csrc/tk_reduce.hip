@@ -75,21 +75,46 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;
     case kFinRatioMax: sc->ratio_scale = m; break;
-    case kFinOcVolume: {   // the candidate was formed at lam = (lo + hi) / 2
+    case kFinOcVolume: {   // the candidate was formed at sc->lam, standing for sc->pend steps
       const double V = s[0];
-      const double lam = 0.5 * (sc->lo + sc->hi);
-      sc->lam = lam;
+      const double lam = sc->lam;
+      const int k = sc->pend;
       sc->V = V;
-      const double err = fabs(V - sc->target);
-      if (err < sc->best_err) {
-        sc->best_err = err;
-        sc->best_lam = lam;
+      if (k > 1 && V > sc->target) {
+        // probe above the target: the k midpoints stay unknown, bisect plainly
+        sc->gallop = 0;
+      } else {
+        // k steps settled: each of the k - 1 midpoints skipped has V at least
+        // as far below the target as this one
+        const double err = fabs(V - sc->target);
+        if (err < sc->best_err) {
+          sc->best_err = err;
+          sc->best_lam = lam;
+        }
+        if (V > sc->target) {
+          sc->lo = lam;
+          sc->gallop = 0;
+        } else {
+          sc->hi = lam;
+        }
+        sc->steps += k;
+        const double rel = (sc->hi - sc->lo) / fmax(sc->hi + sc->lo, DBL_MIN);
+        if (rel < sc->tol || sc->steps >= sc->max_steps) sc->oc_done = 1;
       }
-      if (V > sc->target) sc->lo = lam;
-      else sc->hi = lam;
-      sc->steps += 1;
-      const double rel = (sc->hi - sc->lo) / fmax(sc->hi + sc->lo, DBL_MIN);
-      if (rel < sc->tol || sc->steps >= sc->max_steps) sc->oc_done = 1;
+      if (!sc->oc_done) {
+        // next evaluation: the plain midpoint, or the k-th of the next midpoints
+        int kn = 1;
+        if (sc->gallop && sc->lo == 0.0) kn = (k > 1 && V > sc->target) ? 1 : 2 * k;
+        if (kn > sc->max_steps - sc->steps) kn = sc->max_steps - sc->steps;
+        if (kn < 1) kn = 1;
+        double h = sc->hi, m = h;
+        for (int j = 0; j < kn; ++j) {
+          m = 0.5 * (sc->lo + h);
+          h = m;
+        }
+        sc->lam = m;
+        sc->pend = kn;
+      }
       break;
     }
     case kFinStats:
