@@ -79,7 +79,7 @@ static void free_plan(Plan* P) {
   f(P->r64); f(P->p64); f(P->q64); f(P->x64);
   f(P->r32); f(P->p32); f(P->q32); f(P->x32);
   f(P->ework0); f(P->ework1); f(P->ework2); f(P->ework3);
-  f(P->partials); f(P->counter); f(P->sc);
+  f(P->partials); f(P->sc);
   if (P->sc_host) (void)hipHostFree(P->sc_host);
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -231,7 +231,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
         dmalloc(&P->ework3, g.nele);
   const long long big = g.ndof > g.nele ? g.ndof : g.nele;
   P->partials_len = 4 * (nblocks(big) + 16);
-  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->counter, 1) | dmalloc(&P->sc, 1);
+  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->sc, 1);
   if (rc) {
     free_plan(P);
     return 2;
@@ -252,7 +252,6 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   e = e ? e : hipMemcpy(P->sten_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
   e = e ? e : hipMemcpy(P->sten_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
   e = e ? e : hipMemcpy(P->sc, &zero, sizeof(Scalars), hipMemcpyHostToDevice);
-  e = e ? e : hipMemset(P->counter, 0, sizeof(unsigned));
   e = e ? e : hipMemset(P->eflags, 0, g.nele);
   for (int l = 0; l < nl && e == hipSuccess; ++l)
     e = hipMemset(P->lv[l].fixed, 0, P->lv[l].g.nnode);

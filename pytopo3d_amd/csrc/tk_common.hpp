@@ -111,10 +111,6 @@ __device__ __forceinline__ double block_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Linear block id and block count of a 1-, 2- or 3-D grid.
 __device__ __forceinline__ int block_id() {
   return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
@@ -133,13 +129,13 @@ __device__ __forceinline__ int xcd_logical(int bid, int nb) {
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-// Two-phase grid reductions (the fast path): every block writes its partial
-// value(s) to partials[i * G + block] with no atomics, and a one-block
-// finalize kernel (tk_reduce.hip) combines them in a fixed order and applies
-// the scalar update.  A single device-scope counter touched by every block
-// serialises at the memory side and costs tens of nanoseconds per block, so
-// the counter-based "last block" variant below is kept only for launches with
-// very few blocks.  All threads of the block must call (no early return).
+// Two-phase grid reductions: every block writes its partial value(s) to
+// partials[i * G + block] with no atomics, and a one-block finalize kernel
+// (tk_reduce.hip) combines them in a fixed order and applies the scalar
+// update.  (A single device-scope counter touched by every block, to let the
+// last block finish the sum, serialises at the memory side at tens of
+// nanoseconds per block.)  All threads of the block must call (no early
+// return).
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
   block_sum<NV>(v);
@@ -153,65 +149,6 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
 __device__ __forceinline__ void block_partial_max(double v, double* partials, int slot) {
   v = block_max(v);
   if (threadIdx.x == 0) partials[slot * block_count() + block_id()] = v;
-}
-
-// Grid-wide sum of NV values.  Every thread of every block must call it (no
-// early returns before it).  The last block to finish calls fin(total) from
-// thread 0 and re-arms the counter.  partials holds NV * (number of blocks)
-// doubles.
-template <int NV, class Fin>
-__device__ __forceinline__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter,
-                                         Fin fin) {
-  block_sum<NV>(v);
-  __shared__ int am_last;
-  const int G = block_count();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) partials[i * G + block_id()] = v[i];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == (unsigned)G - 1);
-  }
-  __syncthreads();
-  if (am_last) {
-    __threadfence();
-    double s[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (int b = threadIdx.x; b < G; b += blockDim.x)
-#pragma unroll
-      for (int i = 0; i < NV; ++i) s[i] += load_coherent(partials + i * G + b);
-    block_sum<NV>(s);
-    if (threadIdx.x == 0) {
-      *counter = 0u;
-      fin(s);
-    }
-  }
-}
-
-template <class Fin>
-__device__ __forceinline__ void grid_max(double v, double* partials, unsigned* counter, Fin fin) {
-  v = block_max(v);
-  __shared__ int am_last;
-  const int G = block_count();
-  if (threadIdx.x == 0) {
-    partials[block_id()] = v;
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == (unsigned)G - 1);
-  }
-  __syncthreads();
-  if (am_last) {
-    __threadfence();
-    double s = -INFINITY;
-    for (int b = threadIdx.x; b < G; b += blockDim.x)
-      s = fmax(s, load_coherent(partials + b));
-    s = block_max(s);
-    if (threadIdx.x == 0) {
-      *counter = 0u;
-      fin(s);
-    }
-  }
 }
 
 inline int nblocks(long long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }

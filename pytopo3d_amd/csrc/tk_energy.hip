@@ -16,7 +16,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(
     const double* __restrict__ rho, const double* __restrict__ drho,
     const double* __restrict__ Hs, const uint8_t* __restrict__ fl, double penal, double c0,
     double* __restrict__ ce_out, double* __restrict__ dc_out, double* __restrict__ wc,
-    double* __restrict__ wv, double* partials, unsigned* counter, Scalars* sc) {
+    double* __restrict__ wv, double* partials, Scalars* sc) {
   __shared__ double ke[576];
   for (int i = threadIdx.x; i < 576; i += blockDim.x) ke[i] = KE[i];
   __syncthreads();
@@ -60,7 +60,7 @@ void launch_energy(Plan& P, const double* U, const double* rho, const double* dr
                    double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s) {
   const double c0 = -penal * (1.0 - kEminE);
   hipLaunchKernelGGL(k_energy, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.KE64, U, rho,
-                     drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials, P.counter,
+                     drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials,
                      P.sc);
   finalize(P, kFinC, nblocks(P.g.nele), s);
 }

@@ -76,8 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_transpose2(
 __global__ __launch_bounds__(kBlock) void k_free_sum_projected(
     Grid g, int nsten, const int* __restrict__ off, const double* __restrict__ w,
     const double* __restrict__ Hs, const uint8_t* __restrict__ fl, const double* __restrict__ x,
-    double beta, double eta, double tb_eta, double den, int slot, double* partials,
-    unsigned* counter, Scalars* sc) {
+    double beta, double eta, double tb_eta, double den, int slot, double* partials, Scalars* sc) {
   __shared__ int s_off[kMaxStencil];
   __shared__ double s_w[kMaxStencil];
   stage_stencil(nsten, off, w, s_off, s_w);
@@ -108,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_design(int n, const uint8_t* 
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_free(int n, const uint8_t* __restrict__ fl,
-                                                        double* partials, unsigned* counter,
+                                                        double* partials,
                                                         Scalars* sc) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {(e < n && fl[e] == 0) ? 1.0 : 0.0};
@@ -154,7 +153,7 @@ void launch_free_sum_projected(Plan& P, const double* x, double beta, double eta
   }
   hipLaunchKernelGGL(k_free_sum_projected, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g,
                      P.nsten, P.sten_off, P.sten_w, P.Hs, P.eflags, x, beta, eta, tb_eta, den,
-                     slot, P.partials, P.counter, P.sc);
+                     slot, P.partials, P.sc);
   finalize(P, slot == 0 ? kFinSum0 : kFinSum1, nblocks(P.g.nele), s);
 }
 
@@ -165,7 +164,7 @@ void launch_bounds_design(Plan& P, int upper, double* out, hipStream_t s) {
 
 void launch_count_free(Plan& P, hipStream_t s) {
   hipLaunchKernelGGL(k_count_free, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
-                     P.eflags, P.partials, P.counter, P.sc);
+                     P.eflags, P.partials, P.sc);
   finalize(P, kFinSum0, nblocks(P.g.nele), s);
 }
 

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_ratio(int n, const uint8_t* __res
                                                       const double* __restrict__ dc,
                                                       const double* __restrict__ dv,
                                                       double* __restrict__ ratio,
-                                                      double* partials, unsigned* counter,
+                                                      double* partials,
                                                       Scalars* sc) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double m = -INFINITY;
@@ -85,8 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_volume(Grid g, int nsten, const i
                                                        const uint8_t* __restrict__ fl,
                                                        const double* __restrict__ cand,
                                                        double beta, double eta, double tb_eta,
-                                                       double den, double* partials,
-                                                       unsigned* counter, Scalars* sc) {
+                                                       double den, double* partials, Scalars* sc) {
   if (sc->oc_done) return;
   __shared__ int s_off[kMaxStencil];
   __shared__ double s_w[kMaxStencil];
@@ -109,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_design_stats(int n, const uint8_t* _
                                                           const double* __restrict__ xo,
                                                           const double* __restrict__ xn,
                                                           const double* __restrict__ rho,
-                                                          double* partials, unsigned* counter,
+                                                          double* partials,
                                                           Scalars* sc) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
@@ -133,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_design_stats(int n, const uint8_t* _
 // ---------------------------------------------------------------------------
 void launch_oc_ratio(Plan& P, const double* dc, const double* dv, double* ratio, hipStream_t s) {
   hipLaunchKernelGGL(k_oc_ratio, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele, P.eflags,
-                     dc, dv, ratio, P.partials, P.counter, P.sc);
+                     dc, dv, ratio, P.partials, P.sc);
   finalize(P, kFinRatioMax, nblocks(P.g.nele), s);
 }
 
@@ -166,14 +165,14 @@ void launch_oc_step(Plan& P, const double* x, const double* ratio, double move, 
   }
   hipLaunchKernelGGL(k_oc_volume, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.nsten,
                      P.sten_off, P.sten_w, P.Hs, P.eflags, cand, beta, eta, tb_eta, den,
-                     P.partials, P.counter, P.sc);
+                     P.partials, P.sc);
   finalize(P, kFinOcVolume, nblocks(P.g.nele), s);
 }
 
 void launch_design_stats(Plan& P, const double* x_old, const double* x_new, const double* rho,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_design_stats, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
-                     P.eflags, x_old, x_new, rho, P.partials, P.counter, P.sc);
+                     P.eflags, x_old, x_new, rho, P.partials, P.sc);
   finalize(P, kFinStats, nblocks(P.g.nele), s);
 }
 

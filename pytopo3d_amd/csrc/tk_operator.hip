@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(Grid g, const T* __restrict__ 
                                                    const uint8_t* __restrict__ fixed,
                                                    const T* __restrict__ u, T* __restrict__ y,
                                                    int with_dot, const int* __restrict__ done,
-                                                   double* partials, unsigned* counter,
+                                                   double* partials,
                                                    Scalars* sc) {
   if (done && *done) return;
   __shared__ T ke[576];
@@ -487,8 +487,7 @@ void launch_apply(Plan& P, const Level& L, const T* sl, const T* u, T* y, double
     return;
   }
   hipLaunchKernelGGL(k_apply<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g, ke_of<T>(P), sl,
-                     L.fixed, u, y, with_dot, with_dot ? done_ptr(P) : nullptr, P.partials,
-                     P.counter, P.sc);
+                     L.fixed, u, y, with_dot, with_dot ? done_ptr(P) : nullptr, P.partials, P.sc);
   if (with_dot) finalize(P, kFinAlpha, nblocks(L.g.nnode), s);
 }
 

@@ -91,7 +91,6 @@ struct Plan {
   // reductions
   double* partials = nullptr;
   int partials_len = 0;
-  unsigned* counter = nullptr;
   Scalars* sc = nullptr;          // device scalars
   Scalars* sc_host = nullptr;     // pinned mirror
   Options opt;
