@@ -160,6 +160,9 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     while ((long long)(nmax + 1) * (nmax + 2) / 2 * 8 + 256 <= shm) ++nmax;
     if (P->opt.coarse_max_dof > nmax) P->opt.coarse_max_dof = nmax;
     P->lds_limit = shm;
+    if (hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        P->ncu <= 0)
+      P->ncu = 256;
   }
   P->g = make_grid(nelx, nely, nelz);
   std::memcpy(P->KE, KE, sizeof(double) * 576);

@@ -568,21 +568,50 @@ template <typename T> static const T* sectors_of(const Plan& P);
 template <> const double* sectors_of<double>(const Plan& P) { return P.sec64_dev; }
 template <> const float* sectors_of<float>(const Plan& P) { return P.sec32_dev; }
 
-static void el_grid(const Grid& g, int& nty, int& zc, dim3& grid) {
+// Launch shape of an element sweep: 15x15-node column tiles in x-y, cut into
+// z chunks.  A workgroup streams zc node planes and re-evaluates one element
+// layer, so its time grows as zc + 1, and the workgroups run in rounds of
+// `slots` (compute units x resident workgroups per CU of the kernel).  The
+// chunk count minimises rounds x (zc + 1): a last round that is only partly
+// filled costs as much as a full one.  On the fine grid this gives one or two
+// full rounds of 11-22 planes; on the coarse grids, whose launches are
+// latency-bound, chunks of 1-3 planes.
+static void el_grid(const Grid& g, int slots, long long max_blocks, int& nty, int& zc,
+                    dim3& grid) {
   nty = (g.nny + kOut - 1) / kOut;
   const int ntx = (g.nnx + kOut - 1) / kOut;
-  // z chunks: about 2048 workgroups.  On the fine grid that streams ~10 node
-  // planes per workgroup (one re-evaluated element layer per chunk); on the
-  // coarse grids, whose launches are latency-bound, chunks shrink to 1-2
-  // planes so that the serial z-loop stays short.
-  const int tiles = nty * ntx;
-  int nch = (2048 + tiles - 1) / tiles;
-  if (nch > g.nnz) nch = g.nnz;
-  if (nch < 1) nch = 1;
-  const int zc0 = (g.nnz + nch - 1) / nch;
-  nch = (g.nnz + zc0 - 1) / zc0;
-  zc = (g.nnz + nch - 1) / nch;
-  grid = dim3(tiles, nch);
+  const long long tiles = (long long)nty * ntx;
+  if (slots < 1) slots = 1;
+  long long best = -1;
+  int best_zc = g.nnz, best_n = 1;
+  for (int z = 1; z <= g.nnz; ++z) {
+    const int n = (g.nnz + z - 1) / z;
+    if (n > 1 && tiles * n > max_blocks) continue;
+    const long long rounds = (tiles * n + slots - 1) / slots;
+    const long long cost = rounds * (z + 1);
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_zc = z;
+      best_n = n;
+    }
+  }
+  zc = best_zc;
+  grid = dim3((unsigned)tiles, (unsigned)best_n);
+}
+
+// resident workgroups per compute unit of one element-kernel instantiation
+template <typename T, int MODE>
+static int el_resident() {
+  static int v = 0;
+  if (v == 0) {
+    constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
+    int n = 0;
+    const void* fn = w3 ? (const void*)k_apply_el_w3<T, w3 ? MODE : kModeApply>
+                        : (const void*)k_apply_el<T, w3 ? kModeResidual : MODE>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+    v = n;
+  }
+  return v;
 }
 
 template <typename T, int MODE>
@@ -590,13 +619,8 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
                       int with_dot, const int* done, hipStream_t s) {
   int nty, zc;
   dim3 grid;
-  el_grid(L.g, nty, zc, grid);
   // the reduction scratch holds 4 values per block of the largest 1-D launch
-  while ((long long)grid.x * grid.y > P.partials_len / 4 && grid.y > 1) {
-    const int nch = (int)grid.y - 1;
-    zc = (L.g.nnz + nch - 1) / nch;
-    grid.y = (L.g.nnz + zc - 1) / zc;
-  }
+  el_grid(L.g, P.ncu * el_resident<T, MODE>(), P.partials_len / 4, nty, zc, grid);
   SparseKE<float> kv;
   for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
   constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;

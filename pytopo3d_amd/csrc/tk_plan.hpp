@@ -60,6 +60,7 @@ struct Plan {
   bool kd_ok = false;             // ... equal at all 8 corners (diagonal formed on the fly)
   double* sec64_dev = nullptr;    // device copies (staged into LDS by the element kernel)
   float* sec32_dev = nullptr;
+  int ncu = 0;                    // compute units of the plan's device
   // density filter
   double rmin = 0.0;
   int nsten = 0, R = 0;
