@@ -53,10 +53,10 @@ struct Epilogue {
   T* xo;           // MODE 2..4 updated iterate
   T* d;            // MODE 2..4 new Chebyshev direction (nullptr: not stored, last step)
   const T* dp;     // MODE 3 previous Chebyshev direction
-  const T* dinv;   // MODE 2..4 inverse diagonal; MODE 2/3 nullptr: formed from kd and the
+  const T* dinv;   // MODE 2..4 inverse diagonal; MODE 2/3 nullptr: formed from kdi and the
                    // element scales (KE's diagonal is the same at every corner)
   T c0, c1, c2;
-  T kd[3];         // KE diagonal per component
+  T kdi[3];        // reciprocal KE diagonal per component
   // MODE 6: coarse-grid correction, prolongated on the fly
   const T* xc;
   Grid cg;
@@ -179,6 +179,11 @@ __device__ __forceinline__ void sector_apply(const T* __restrict__ k, T s, const
     o[2][3] = s * (b[6] * x0 + b[7] * x1 + b[8] * x2);
   }
 }
+
+// Reciprocal of the on-the-fly diagonal: the hardware reciprocal in FP32
+// (1 ulp; it only scales the smoother's correction), a rounded division in FP64.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
 template <typename T, int MODE>
 struct PlaneRegs {
@@ -414,8 +419,9 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
       const T scur = se + ss[tid - 1] + ss[tid - kTile] + ss[tid - kTile - 1];
       const T dsum = s_prev + scur;
       s_prev = scur;
+      const T inv = recip(dsum);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) edi[c] = ((fix_prev >> c) & 1) ? T(1) : T(1) / (ep.kd[c] * dsum);
+      for (int c = 0; c < 3; ++c) edi[c] = ((fix_prev >> c) & 1) ? T(1) : inv * ep.kdi[c];
     }
     // node (ly, lx) of plane ez: own corner f = 0, element (ly-1, lx) its f = 2
     // (oy = 1), element (ly, lx-1) its f = 1, element (ly-1, lx-1) its f = 3
@@ -678,7 +684,7 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
   ep.dinv = P.kd_ok ? nullptr : dinv;
-  for (int c = 0; c < 3; ++c) ep.kd[c] = (T)P.kd[c];
+  for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.c1 = (T)c1;
   ep.c2 = (T)c2;
@@ -697,7 +703,7 @@ void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
   ep.dinv = P.kd_ok ? nullptr : dinv;
-  for (int c = 0; c < 3; ++c) ep.kd[c] = (T)P.kd[c];
+  for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.xc = xc;
   ep.cg = C.g;
