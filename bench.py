@@ -52,7 +52,10 @@ def _parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--smooth-degree", type=int, default=0, help="Chebyshev degree (0: default)")
+    ap.add_argument("--smooth-degree", type=int, default=0,
+                    help="Chebyshev degree on the two finest levels (0: default)")
+    ap.add_argument("--coarse-degree", type=int, default=0,
+                    help="Chebyshev degree on the coarser levels (0: default)")
     ap.add_argument("--cheb-lo", type=float, default=0.0, help="smoothing interval low end (0: default)")
     ap.add_argument("--operator-kind", type=int, default=0,
                     help="0: element/sector kernel (default), 1: dense node kernel")
@@ -77,10 +80,12 @@ def _setup(args):
     solid, void, free = protected_masks(nelx, nely, nelz, None, None, None, None, F)
     KE = element_stiffness(None, 1.0)
     fixed = support_node_flags(nelx, nely, nelz, None)
+    # options left at 0 on the command line keep the library defaults (and
+    # are reported as such)
+    opts = {"operator_kind": args.operator_kind, "smooth_degree": args.smooth_degree,
+            "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo}
     problem = SIMPProblem(nelx, nely, nelz, 3.0, KE, F, fixed, solid, void,
-                          solver_options={"operator_kind": args.operator_kind,
-                                          "smooth_degree": args.smooth_degree,
-                                          "cheb_lo": args.cheb_lo})
+                          solver_options={k: v for k, v in opts.items() if v})
     x0 = np.full((nely, nelx, nelz), 0.3)
     x0[solid] = 1.0
     x0[void] = 0.0
@@ -297,7 +302,8 @@ def main():
             "nelx": args.nelx, "nely": args.nely, "nelz": args.nelz,
             "volfrac": 0.3, "penal": 3.0, "rmin": 3.0, "beta": 1.0, "eta": 0.5, "move": 0.2,
             "solver_rtol": args.rtol, "levels": plan.levels(),
-            "smooth_degree": plan.options["smooth_degree"], "cheb_lo": plan.options["cheb_lo"],
+            "smooth_degree": plan.options["smooth_degree"],
+            "coarse_degree": plan.options["coarse_degree"], "cheb_lo": plan.options["cheb_lo"],
         },
         "solve_iters_mean": float(np.mean([r["solve_iters"] for r in recs_m])),
         "oc_steps_mean": float(np.mean([r["oc_steps"] for r in recs_m])),

@@ -64,6 +64,9 @@ typedef struct tk_options {
   int filter_kind;        /* density filter kernel: 0 LDS tiles when the stencil radius is
                              <= 3 (default), 1 stencil gather (the only form for larger
                              radii) */
+  int coarse_degree;      /* Chebyshev smoothing steps per pre/post smoothing on the third
+                             and coarser multigrid levels (default 4); smooth_degree applies
+                             to the two finest */
 } tk_options;
 
 /* Fill *opt with the defaults. */

@@ -43,6 +43,7 @@ class TkOptions(ctypes.Structure):
         ("operator_kind", ctypes.c_int),
         ("no_graphs", ctypes.c_int),
         ("filter_kind", ctypes.c_int),
+        ("coarse_degree", ctypes.c_int),
     ]
 
 

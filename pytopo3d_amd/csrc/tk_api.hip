@@ -134,6 +134,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   Plan* P = new Plan();
   if (o) {
     if (o->smooth_degree > 0) P->opt.smooth_degree = o->smooth_degree;
+    if (o->coarse_degree > 0) P->opt.coarse_degree = o->coarse_degree;
     if (o->cheb_lo > 0.0) P->opt.cheb_lo = o->cheb_lo;
     if (o->cheb_hi > 0.0) P->opt.cheb_hi = o->cheb_hi;
     if (o->max_levels > 0) P->opt.max_levels = o->max_levels < kMaxLevels ? o->max_levels : kMaxLevels;
@@ -340,6 +341,7 @@ void tk_options_default(tk_options* o) {
   o->operator_kind = d.operator_kind;
   o->no_graphs = d.no_graphs;
   o->filter_kind = d.filter_kind;
+  o->coarse_degree = d.coarse_degree;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

@@ -29,7 +29,8 @@ struct Level {
 };
 
 struct Options {
-  int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing
+  int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing, two finest levels
+  int coarse_degree = 4;         // ... on the third and coarser levels
   double cheb_lo = 0.06;         // smoothing interval [cheb_lo, cheb_hi] * lambda_max bound
   double cheb_hi = 1.0;
   int max_levels = kMaxLevels;

@@ -222,20 +222,23 @@ template <typename T>
 static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s,
                  const T* xc = nullptr) {
   Level& L = P.lv[l];
+  // the coarse levels are latency-bound: more steps there cost little and
+  // strengthen the cycle
+  const int deg = l >= 2 ? P.opt.coarse_degree : P.opt.smooth_degree;
   LevelBuf<T>& B = buf<T>(L);
   const double lmax = P.lam_bound * P.opt.cheb_hi, lmin = P.lam_bound * P.opt.cheb_lo;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   int k = 1;
-  if (x_zero && P.use_el && P.opt.smooth_degree >= 2) {
+  if (x_zero && P.use_el && deg >= 2) {
     // steps 1 and 2 from x = 0 in one sweep
     const double rho_new = 1.0 / (2.0 * sigma - rho_old);
     if (&L == &P.lv[0]) {
       if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
     }
     el_cheb_zero2<T>(P, L, B.s, B.dinv, b, xa, B.d, 1.0 / theta, rho_new * rho_old,
-                     2.0 * rho_new / delta, P.opt.smooth_degree > 2, &P.sc->done, s);
+                     2.0 * rho_new / delta, deg > 2, &P.sc->done, s);
     std::swap(x, xa);
     rho_old = rho_new;
     k = 2;
@@ -244,17 +247,17 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
       if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
     }
     el_cheb_prolong<T>(P, L, P.lv[l + 1], xc, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta,
-                       P.opt.smooth_degree > 1, &P.sc->done, s);
+                       deg > 1, &P.sc->done, s);
     std::swap(x, xa);
   } else {
     launch_cheb_first<T>(P, L, B.s, B.dinv, b, x, xa, B.d, 1.0 / theta, x_zero,
-                         P.opt.smooth_degree > 1, s);
+                         deg > 1, s);
     std::swap(x, xa);
   }
-  for (; k < P.opt.smooth_degree; ++k) {
+  for (; k < deg; ++k) {
     const double rho_new = 1.0 / (2.0 * sigma - rho_old);
     launch_cheb_step<T>(P, L, B.s, B.dinv, b, x, xa, B.d, rho_new * rho_old, 2.0 * rho_new / delta,
-                        k + 1 < P.opt.smooth_degree, s);
+                        k + 1 < deg, s);
     std::swap(x, xa);
     rho_old = rho_new;
   }
