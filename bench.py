@@ -304,6 +304,7 @@ def main():
             "solver_rtol": args.rtol, "levels": plan.levels(),
             "smooth_degree": plan.options["smooth_degree"],
             "coarse_degree": plan.options["coarse_degree"], "cheb_lo": plan.options["cheb_lo"],
+            "coarse_cheb_lo": plan.options["coarse_cheb_lo"],
         },
         "solve_iters_mean": float(np.mean([r["solve_iters"] for r in recs_m])),
         "oc_steps_mean": float(np.mean([r["oc_steps"] for r in recs_m])),

@@ -44,6 +44,7 @@ class TkOptions(ctypes.Structure):
         ("no_graphs", ctypes.c_int),
         ("filter_kind", ctypes.c_int),
         ("coarse_degree", ctypes.c_int),
+        ("coarse_cheb_lo", ctypes.c_double),
     ]
 
 

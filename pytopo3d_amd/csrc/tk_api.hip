@@ -135,6 +135,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   if (o) {
     if (o->smooth_degree > 0) P->opt.smooth_degree = o->smooth_degree;
     if (o->coarse_degree > 0) P->opt.coarse_degree = o->coarse_degree;
+    if (o->coarse_cheb_lo > 0.0) P->opt.coarse_cheb_lo = o->coarse_cheb_lo;
     if (o->cheb_lo > 0.0) P->opt.cheb_lo = o->cheb_lo;
     if (o->cheb_hi > 0.0) P->opt.cheb_hi = o->cheb_hi;
     if (o->max_levels > 0) P->opt.max_levels = o->max_levels < kMaxLevels ? o->max_levels : kMaxLevels;
@@ -145,9 +146,9 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->check_every > 0) P->opt.check_every = o->check_every;
     if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
   }
-  if (!(P->opt.cheb_lo < P->opt.cheb_hi)) {
+  if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
     delete P;
-    set_error("cheb_lo must be below cheb_hi");
+    set_error("cheb_lo and coarse_cheb_lo must be below cheb_hi");
     return 1;
   }
   {
@@ -342,6 +343,7 @@ void tk_options_default(tk_options* o) {
   o->no_graphs = d.no_graphs;
   o->filter_kind = d.filter_kind;
   o->coarse_degree = d.coarse_degree;
+  o->coarse_cheb_lo = d.coarse_cheb_lo;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

@@ -222,11 +222,13 @@ template <typename T>
 static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s,
                  const T* xc = nullptr) {
   Level& L = P.lv[l];
-  // the coarse levels are latency-bound: more steps there cost little and
-  // strengthen the cycle
-  const int deg = l >= 2 ? P.opt.coarse_degree : P.opt.smooth_degree;
+  // the coarse levels are latency-bound: a longer smoother over a wider
+  // interval costs little there and strengthens the cycle
+  const bool coarse = l >= 2;
+  const int deg = coarse ? P.opt.coarse_degree : P.opt.smooth_degree;
   LevelBuf<T>& B = buf<T>(L);
-  const double lmax = P.lam_bound * P.opt.cheb_hi, lmin = P.lam_bound * P.opt.cheb_lo;
+  const double lmax = P.lam_bound * P.opt.cheb_hi;
+  const double lmin = P.lam_bound * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
