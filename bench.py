@@ -308,6 +308,7 @@ def main():
         },
         "solve_iters_mean": float(np.mean([r["solve_iters"] for r in recs_m])),
         "oc_steps_mean": float(np.mean([r["oc_steps"] for r in recs_m])),
+        "oc_evals_mean": float(np.mean([r["oc_evals"] for r in recs_m])),
         "work_timed_mixed": work_m,
         "phase_ms_mixed": {k: 1e3 * float(np.mean([r[k] for r in recs_m]))
                            for k in ("t_solve", "t_sens", "t_oc")},

@@ -186,7 +186,8 @@ int tk_volume_bounds(tk_plan* plan, double beta, double eta, double* vmin, doubl
 
 /* Synchronise and copy the device results (HOST out[0..n)):
  * [0] c, [1] change, [2] vol_sum, [3] gray count, [4] OC steps, [5] OC lambda,
- * [6] OC volume, [7] OC ratio scale, [8] nfree. */
+ * [6] OC volume (of the last evaluation), [7] OC ratio scale, [8] nfree,
+ * [9] OC volume evaluations. */
 int tk_read_results(tk_plan* plan, double* out, int n, void* stream);
 
 /* Work counters (HOST out[0..n)): [0] FP64 fine operator applications,

@@ -257,10 +257,10 @@ class Plan:
         return float(lo.value), float(hi.value)
 
     def results(self) -> dict:
-        out = (_D * 9)()
-        _check(lib().tk_read_results(self._h, out, 9, _stream()))
+        out = (_D * 10)()
+        _check(lib().tk_read_results(self._h, out, 10, _stream()))
         keys = ("c", "change", "vol_sum", "gray_cnt", "oc_steps", "oc_lambda", "oc_volume",
-                "ratio_scale", "nfree")
+                "ratio_scale", "nfree", "oc_evals")
         return dict(zip(keys, list(out)))
 
     def counters(self) -> dict:

@@ -2,8 +2,10 @@
 // lifecycle, argument validation with the reference's error messages, and the
 // entry points that enqueue the kernels of the other translation units.
 #include "pytopo3d_hip.h"
+#include "tk_bisect.hpp"
 #include "tk_plan.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -18,7 +20,7 @@ const char* last_error() { return g_err.c_str(); }
 int apply_fine(Plan& P, int prec, const void* u, void* y, hipStream_t s);
 int smooth_step_fine(Plan& P, int prec, const void* b, const void* x, void* xo, void* d,
                      double c1, double c2, hipStream_t s);
-void launch_oc_init(Plan& P, double target, int max_steps, double tol, hipStream_t s);
+void launch_oc_init(Plan& P, double target, int max_steps, double tol, int probes, hipStream_t s);
 void launch_chain_weights(Plan& P, const double* gc, const double* gv, const double* drho,
                           double* wc, double* wv, hipStream_t s);
 
@@ -301,6 +303,27 @@ static void proj_consts(double beta, double eta, double& tb, double& den) {
 static int read_sc(Plan& P, hipStream_t s) {
   TK_HIP(hipMemcpyAsync(P.sc_host, P.sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   TK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Enqueue OC evaluations in chunks until the device bisection has stopped.
+// Every evaluation measures a midpoint (at most max_steps), is one of at most
+// kOcMaxProbes probes or one of at most kOcMaxChecks checks after the last
+// step, so the cap is never the stop.
+static int oc_bisect(Plan& P, const double* x, double move, double beta, double eta, double tb,
+                     double den, int max_steps, hipStream_t s) {
+  const int cap = max_steps + kOcMaxProbes + kOcMaxChecks;
+  int launched = 0, chunk = 12;
+  for (;;) {
+    const int n = std::min(chunk, cap - launched);
+    for (int k = 0; k < n; ++k)
+      launch_oc_step(P, x, P.ework0, move, P.ework1, beta, eta, tb, den, s);
+    launched += n;
+    if (read_sc(P, s)) return 2;
+    if (P.sc_host->oc_done || launched >= cap) break;
+    chunk = 8;
+  }
+  P.cnt.oc_steps += P.sc_host->evals;
   return 0;
 }
 
@@ -616,17 +639,16 @@ int tk_oc_update(tk_plan* plan, const double* x, const double* dc_dx, const doub
   tk::proj_consts(beta, eta, tb, den);
   tk::launch_oc_ratio(P, dc_dx, dv_dx, P.ework0, s);
   tk::launch_oc_normalize(P, P.ework0, s);
-  tk::launch_oc_init(P, target, max_steps, tol, s);
-  // one more evaluation than steps: a probe that overshoots settles none
-  for (int k = 0; k <= max_steps; ++k) {
-    tk::launch_oc_step(P, x, P.ework0, move, P.ework1, beta, eta, tb, den, s);
+  tk::launch_oc_init(P, target, max_steps, tol, 1, s);
+  if (tk::oc_bisect(P, x, move, beta, eta, tb, den, max_steps, s)) return 2;
+  if (P.sc_host->redo) {
+    // a midpoint decided unmeasured might have been the kept one: measure all
+    tk::launch_oc_init(P, target, max_steps, tol, 0, s);
+    if (tk::oc_bisect(P, x, move, beta, eta, tb, den, max_steps, s)) return 2;
   }
   tk::launch_oc_candidate(P, x, P.ework0, move, 1, x_new, s);
   TK_LAUNCH_CHECK();
-  if (steps_out) {
-    if (tk::read_sc(P, s)) return 2;
-    *steps_out = P.sc_host->steps;
-  }
+  if (steps_out) *steps_out = P.sc_host->steps;
   return 0;
 }
 
@@ -665,9 +687,9 @@ int tk_read_results(tk_plan* plan, double* out, int n, void* stream) {
   TK_PLAN_OR_FAIL(plan);
   if (tk::read_sc(P, (hipStream_t)stream)) return 2;
   const tk::Scalars& S = *P.sc_host;
-  const double v[9] = {S.c, S.change, S.vol_sum, S.gray_cnt, (double)S.steps, S.best_lam,
-                       S.V, S.ratio_scale, (double)P.nfree};
-  for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
+  const double v[10] = {S.c, S.change, S.vol_sum, S.gray_cnt, (double)S.steps, S.best_lam,
+                        S.V, S.ratio_scale, (double)P.nfree, (double)S.evals};
+  for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return 0;
 }
 

@@ -53,8 +53,14 @@ struct Scalars {
   int iter, done, maxit, stalled;
   // optimality-criteria bisection
   double lo, hi, lam, best_lam, best_err, target, V, ratio_scale, tol;
-  int steps, oc_done, max_steps, pend;   // pend: bisection steps the next evaluation resolves
-  int gallop, pad1;                      // gallop: the lower bound is still 0, probe ahead
+  double ka, kva, kb, kvb;   // evaluated multipliers nearest the crossing: V(ka) > target >= V(kb)
+  double lam_prev;           // the previous bisection's kept multiplier (first probe)
+  double gfac;               // step factor of the one-sided search
+  double slot_lam[4], slot_v[4];   // last two midpoints on each side of the target
+  int slot_state[4], slot_step[4];
+  int steps, oc_done, max_steps, evals;
+  int spec, nspec, redo, phase;  // probes on, probes made, repeat without probes, checks
+  int check, pad1;               // slot being measured after the last step
   // general results
   double c;            // compliance
   double change;       // max |x_new - x| over free elements

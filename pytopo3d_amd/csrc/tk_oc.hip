@@ -8,14 +8,14 @@
 //   filtered candidate; the lambda with the smallest |V - target| (strict <)
 //   is kept; V > target raises the lower bound; stop when
 //   (hi - lo) / max(hi + lo, tiny) < tol or after max_steps.
-// While the lower bound is still 0 the bisection midpoints are hi/2, hi/4, ...
-// and V is non-increasing in lambda, so one evaluation at the k-th of them
-// that lands at or below the target settles all k steps (each sets hi = mid);
-// the probe length doubles while that holds (see kFinOcVolume), which skips
-// most of the ~40 steps from 1e12 down to the multiplier's scale.
-// The kept candidate is rebuilt from the kept lambda with the same formula
-// (oc_candidate), so it is bitwise the candidate that was measured.
+// Each evaluation is a (candidate, projected volume, finalize) triple; the
+// finalize records V and places the next evaluation (tk_bisect.hpp: midpoints
+// whose decision an earlier evaluation implies are not measured, and a few
+// probes near the crossing make most of them so).  The kept candidate is
+// rebuilt from the kept lambda with the same formula (oc_candidate), so it is
+// bitwise the candidate that was measured.
 // Also: change / volume / gray statistics (pytopo3d/core/optimizer.py:642-655).
+#include "tk_bisect.hpp"
 #include "tk_filter_dev.hpp"
 #include "tk_plan.hpp"
 
@@ -49,20 +49,9 @@ __global__ __launch_bounds__(kBlock) void k_oc_normalize(int n, const uint8_t* _
   ratio[e] = (scale > 0.0) ? ratio[e] / scale : 1.0;
 }
 
-__global__ void k_oc_init(Scalars* sc, double target, int max_steps, double tol) {
+__global__ void k_oc_init(Scalars* sc, double target, int max_steps, double tol, int probes) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  sc->lo = 0.0;
-  sc->hi = 1.0e12;
-  sc->best_err = INFINITY;
-  sc->best_lam = 0.5e12;
-  sc->target = target;
-  sc->steps = 0;
-  sc->oc_done = 0;
-  sc->max_steps = max_steps;
-  sc->tol = tol;
-  sc->pend = 1;
-  sc->gallop = 1;
-  sc->lam = 0.5 * (sc->lo + sc->hi);
+  oc_init(sc, target, max_steps, tol, probes);
 }
 
 __global__ __launch_bounds__(kBlock) void k_oc_candidate(int n, const uint8_t* __restrict__ fl,
@@ -141,8 +130,8 @@ void launch_oc_normalize(Plan& P, double* ratio, hipStream_t s) {
                      P.eflags, ratio, P.sc);
 }
 
-void launch_oc_init(Plan& P, double target, int max_steps, double tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, P.sc, target, max_steps, tol);
+void launch_oc_init(Plan& P, double target, int max_steps, double tol, int probes, hipStream_t s) {
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, P.sc, target, max_steps, tol, probes);
 }
 
 void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double move, int use_best,
@@ -151,13 +140,13 @@ void launch_oc_candidate(Plan& P, const double* x, const double* ratio, double m
                      P.eflags, x, ratio, move, use_best, cand, P.sc);
 }
 
-// One bisection step at lambda = (lo + hi) / 2: the candidate (k_oc_candidate;
-// forming it inside the tiled filter instead measured slower, the square root
-// and division being repeated on the 2.5x larger halo-padded tile), its
-// free-element projected volume, and the bracket update (kFinOcVolume).
+// One evaluation at sc->lam: the candidate (k_oc_candidate; forming it inside
+// the tiled filter instead measured slower, the square root and division being
+// repeated on the 2.5x larger halo-padded tile), its free-element projected
+// volume, and the bisection update (kFinOcVolume).  All three return at once
+// after the bisection has stopped.
 void launch_oc_step(Plan& P, const double* x, const double* ratio, double move, double* cand,
                     double beta, double eta, double tb_eta, double den, hipStream_t s) {
-  P.cnt.oc_steps++;
   launch_oc_candidate(P, x, ratio, move, 0, cand, s);
   if (P.ftiled) {
     tiled_oc_volume(P, cand, beta, eta, tb_eta, den, s);

@@ -2,6 +2,7 @@
 // partials of the previous launch in a fixed order (deterministic) and applies
 // the scalar update of the algorithm step that produced them (CG step lengths
 // and convergence tests, the OC bisection bracket, compliance, statistics).
+#include "tk_bisect.hpp"
 #include "tk_plan.hpp"
 
 #include <cfloat>
@@ -75,48 +76,9 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;
     case kFinRatioMax: sc->ratio_scale = m; break;
-    case kFinOcVolume: {   // the candidate was formed at sc->lam, standing for sc->pend steps
-      const double V = s[0];
-      const double lam = sc->lam;
-      const int k = sc->pend;
-      sc->V = V;
-      if (k > 1 && V > sc->target) {
-        // probe above the target: the k midpoints stay unknown, bisect plainly
-        sc->gallop = 0;
-      } else {
-        // k steps settled: each of the k - 1 midpoints skipped has V at least
-        // as far below the target as this one
-        const double err = fabs(V - sc->target);
-        if (err < sc->best_err) {
-          sc->best_err = err;
-          sc->best_lam = lam;
-        }
-        if (V > sc->target) {
-          sc->lo = lam;
-          sc->gallop = 0;
-        } else {
-          sc->hi = lam;
-        }
-        sc->steps += k;
-        const double rel = (sc->hi - sc->lo) / fmax(sc->hi + sc->lo, DBL_MIN);
-        if (rel < sc->tol || sc->steps >= sc->max_steps) sc->oc_done = 1;
-      }
-      if (!sc->oc_done) {
-        // next evaluation: the plain midpoint, or the k-th of the next midpoints
-        int kn = 1;
-        if (sc->gallop && sc->lo == 0.0) kn = (k > 1 && V > sc->target) ? 1 : 2 * k;
-        if (kn > sc->max_steps - sc->steps) kn = sc->max_steps - sc->steps;
-        if (kn < 1) kn = 1;
-        double h = sc->hi, m = h;
-        for (int j = 0; j < kn; ++j) {
-          m = 0.5 * (sc->lo + h);
-          h = m;
-        }
-        sc->lam = m;
-        sc->pend = kn;
-      }
+    case kFinOcVolume:   // the candidate was formed at sc->lam
+      oc_record(sc, s[0]);
       break;
-    }
     case kFinStats:
       sc->vol_sum = s[0];
       sc->gray_cnt = s[1];

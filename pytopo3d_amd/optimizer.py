@@ -143,6 +143,7 @@ def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float
     st.x, st.x_new = st.x_new, st.x
     return {"c": c, "change": float(res["change"]), "vol": res["vol_sum"] / problem.nfree,
             "gray": res["gray_cnt"] / problem.nfree, "oc_steps": int(res["oc_steps"]),
+            "oc_evals": int(res["oc_evals"]),
             "solve_iters": problem.last_iters, "relres": problem.last_relres,
             "t_solve": t1 - t0, "t_sens": t2 - t1, "t_oc": t3 - t2}
 
