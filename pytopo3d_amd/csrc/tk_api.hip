@@ -172,7 +172,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   std::memcpy(P->KE, KE, sizeof(double) * 576);
   P->lam_bound = lambda_bound(KE);
   operator_setup(*P);
-  P->use_el = P->sector_ok && P->opt.operator_kind == 0;
+  // the element kernel's smoothing forms D^-1 from KE's corner-independent diagonal
+  P->use_el = P->sector_ok && P->kd_ok && P->opt.operator_kind == 0;
   P->rmin = rmin;
   // stencil: integer offsets with rmin - |d| > 1e-9, ordered (dz, dx, dy) ascending
   const int R = (int)std::ceil(rmin);

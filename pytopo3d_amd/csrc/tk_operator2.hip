@@ -53,8 +53,8 @@ struct Epilogue {
   T* xo;           // MODE 2..4 updated iterate
   T* d;            // MODE 2..4 new Chebyshev direction (nullptr: not stored, last step)
   const T* dp;     // MODE 3 previous Chebyshev direction
-  const T* dinv;   // MODE 2..4 inverse diagonal; MODE 2/3 nullptr: formed from kdi and the
-                   // element scales (KE's diagonal is the same at every corner)
+  const T* dinv;   // MODE 4 inverse diagonal (MODE 2, 3, 6 form it from kdi and the element
+                   // scales: KE's diagonal is the same at every corner)
   T c0, c1, c2;
   T kdi[3];        // reciprocal KE diagonal per component
   // MODE 6: coarse-grid correction, prolongated on the fly
@@ -194,6 +194,15 @@ struct PlaneRegs {
   T odi[3];
 };
 
+// Loads of element layer ez: its element scale and the epilogue operands of
+// node plane ez (whose nodes the layer completes).  They are issued one layer
+// ahead, like the node planes, so no layer waits on its own loads.
+template <typename T>
+struct LayerIn {
+  T se;
+  T eb[3], ed[3];
+};
+
 template <typename T, int MODE>
 __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& Kv,
                                          const T* __restrict__ Kg, const T* __restrict__ s,
@@ -201,13 +210,14 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
                                          const T* __restrict__ u, const Epilogue<T>& ep, int nty,
                                          int zc, int with_dot, const int* __restrict__ done,
                                          double* partials) {
-  if (done && *done) return;
   __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked operand)
   __shared__ T yo[3][3][kTile * kTile];   // exchanged corner contributions (f = 1, 2, 3)
   __shared__ T kb[kSecN];                 // FP64 sector entries
   __shared__ T ss[kTile * kTile];         // element scales of the layer (on-the-fly diagonal)
   constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep || MODE == kModeChebFirstP;
-  const bool diag_fly = kCheb && ep.dinv == nullptr;
+  // Chebyshev sweeps form D^-1 from the element scales (the element kernel
+  // is used only when KE's diagonal is the same at every corner)
+  constexpr bool diag_fly = kCheb;
 
   const int tid = threadIdx.x;
   // FP64 entries are staged in LDS (90 SGPRs would spill); FP32 entries stay
@@ -330,10 +340,38 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     for (int c = 0; c < 3; ++c) face_fwd(F[c]);
   };
 
+  // element scale and epilogue operands of layer ez (see LayerIn)
+  constexpr bool kNeedB = MODE == kModeResidual || MODE == kModeChebFirst ||
+                          MODE == kModeChebStep || MODE == kModeChebFirstP;
+  auto fetch_layer = [&](int ez, LayerIn<T>& L) {
+    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
+    L.se = el ? s[ey + ex * g.nely + ez * exy] : T(0);   // 0 gives zero contributions
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L.eb[c] = L.ed[c] = T(0);
+    if constexpr (kNeedB) {
+      if (owner && ez >= Z0) {
+        const int n_out = ey + ex * g.nny + ez * nxy;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          L.eb[c] = ep.b[3 * n_out + c];
+          if (MODE == kModeChebStep) L.ed[c] = ep.dp[3 * n_out + c];
+        }
+      }
+    }
+  };
+
+  // the operator-only sweeps run at higher occupancy, which hides these loads
+  // without the registers of a second set
+  constexpr bool kPrefetch = MODE != kModeApply && MODE != kModeApplyDir;
+
   double dot = 0.0;
   PlaneRegs<T, MODE> RA, RB;
+  LayerIn<T> LA, LB;
   fetch(Z0 - 1, RB);
   fetch(Z0, RA);
+  if constexpr (kPrefetch) fetch_layer(Z0 - 1, LB);
+  // the convergence flag is read with the first loads in flight
+  if (done && *done) return;
   store(0, RB);
   store(1, RA);
   // own node of the plane being finished (the bottom plane of the layer)
@@ -356,31 +394,26 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   // pre receives plane ez + 2.  Consecutive layers swap the roles of the
   // buffer pairs, so the z-loop is unrolled by two and nothing is copied.
   auto layer = [&](int ez, int buf, T (&Fb)[3][4], T (&Ft)[3][4], const PlaneRegs<T, MODE>& nxt,
-                   PlaneRegs<T, MODE>& pre) {
-    // issue the global loads of the next layer and of this epilogue early
+                   PlaneRegs<T, MODE>& pre, const LayerIn<T>& cur, LayerIn<T>& nx) {
+    // issue the global loads of the next layer first
     const bool more = ez + 1 < Z1;
-    if (more) fetch(ez + 2, pre);
+    if (more) {
+      fetch(ez + 2, pre);
+      if constexpr (kPrefetch) fetch_layer(ez + 1, nx);
+    }
+    LayerIn<T> here;
+    if constexpr (!kPrefetch) fetch_layer(ez, here);
+    const LayerIn<T>& lin = kPrefetch ? cur : here;
     const bool out_now = owner && ez >= Z0;
     const int n_out = ey + ex * g.nny + ez * nxy;
-    T eb[3] = {T(0), T(0), T(0)}, ed[3] = {T(0), T(0), T(0)}, edi[3] = {T(0), T(0), T(0)};
-    if constexpr (MODE == kModeChebZero) {
+    T eb[3], ed[3], edi[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        eb[c] = pb_prev[c];
-        edi[c] = pdi_prev[c];
-      }
-    } else if (MODE != kModeApply) {
-      if (out_now) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          eb[c] = ep.b[3 * n_out + c];
-          if (MODE != kModeResidual && !diag_fly) edi[c] = ep.dinv[3 * n_out + c];
-          if (MODE == kModeChebStep) ed[c] = ep.dp[3 * n_out + c];
-        }
-      }
+    for (int c = 0; c < 3; ++c) {
+      eb[c] = (MODE == kModeChebZero) ? pb_prev[c] : lin.eb[c];
+      edi[c] = (MODE == kModeChebZero) ? pdi_prev[c] : T(0);   // else formed below
+      ed[c] = lin.ed[c];
     }
-    const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
-    const T se = el ? s[ey + ex * g.nely + ez * exy] : T(0);   // 0 gives zero contributions
+    const T se = lin.se;
 
     // forward transform, sector blocks, inverse z butterfly
     face_spectrum(buf, Ft);
@@ -467,9 +500,9 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   };
 
   for (int ez = Z0 - 1; ez < Z1; ez += 2) {
-    layer(ez, 1, F0, F1, RA, RB);
+    layer(ez, 1, F0, F1, RA, RB, LB, LA);
     if (ez + 1 >= Z1) break;
-    layer(ez + 1, 0, F1, F0, RB, RA);
+    layer(ez + 1, 0, F1, F0, RB, RA, LA, LB);
   }
   if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot) {
     double v[1] = {dot};
@@ -683,7 +716,6 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   ep.xo = xo;
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
-  ep.dinv = P.kd_ok ? nullptr : dinv;
   for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.c1 = (T)c1;
@@ -702,7 +734,6 @@ void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const
   ep.xo = xo;
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
-  ep.dinv = P.kd_ok ? nullptr : dinv;
   for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.xc = xc;
