@@ -166,7 +166,12 @@ int tk_energy(tk_plan* plan, const double* U, const double* rho, const double* d
 /* Projected-volume OC update (optimality_criteria_update_projected,
  * pytopo3d/utils/oc_update.py:25-124), the bisection entirely on the device.
  * x, dc_dx, dv_dx: nele doubles; x_new: output.  target: free-volume target
- * (a sum over free elements).  steps_out (HOST, may be NULL; synchronises). */
+ * (a sum over free elements).  steps_out (HOST, may be NULL): the bisection
+ * steps, as the reference counts them.  Synchronises the stream: the host
+ * reads the device bisection state between chunks of volume evaluations, which
+ * stop when the bisection has (midpoints that an earlier evaluation decides
+ * through the monotone volume are not measured; the result is the reference's
+ * bisection result). */
 int tk_oc_update(tk_plan* plan, const double* x, const double* dc_dx, const double* dv_dx,
                  double move, double target, double beta, double eta, int max_steps, double tol,
                  double* x_new, int* steps_out, void* stream);
