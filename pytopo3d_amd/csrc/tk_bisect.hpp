@@ -18,7 +18,8 @@
 // halts at the first midpoint inside (ka, kb), which must be measured.
 // oc_next_probe chooses where the next evaluation goes: that midpoint, or a
 // probe meant to pull ka and kb tight around the crossing (the previous call's
-// multiplier, a search outwards from it, then two-sided secant steps in
+// multiplier, a Newton step in log lambda with the previous call's slope, or a
+// search outwards when there is none, then two-sided secant steps in
 // log lambda), after which most remaining midpoints are decided unmeasured.
 //
 // The kept midpoint.  The midpoints visited above the target increase (each
@@ -29,11 +30,12 @@
 // side has exactly the same volume (a plateau: the first of equal ones is
 // kept).  The bisection therefore tracks, per side, the last two midpoints
 // with their volume when measured or its bound when implied; after the last
-// step it measures what is still needed (at most four evaluations: the final
-// lo and hi, and their predecessors when the bound does not separate them
-// strictly).  A plateau at the end, or a NaN volume, makes the host repeat the
-// bisection measuring every midpoint (probes off), which keeps the smallest
-// error directly as the reference does.
+// step it measures what the choice still needs (at most three evaluations:
+// the final lo and hi when a bound cannot decide between them, and the
+// winner's predecessor when its bound does not separate it strictly).  A
+// plateau at the end, an exact tie across the sides or a NaN volume makes the
+// host repeat the bisection measuring every midpoint (probes off), which keeps
+// the smallest error directly as the reference does.
 #pragma once
 
 #include <cfloat>
@@ -44,7 +46,7 @@
 namespace tk {
 
 constexpr int kOcMaxProbes = 24;   // probe evaluations at most
-constexpr int kOcMaxChecks = 4;    // evaluations after the last bisection step at most
+constexpr int kOcMaxChecks = 3;    // evaluations after the last bisection step at most
 
 // Side slots: 0 = final lo, 1 = the midpoint above the target visited before
 // it, 2 = final hi, 3 = the midpoint below the target visited before it.
@@ -134,42 +136,73 @@ __host__ __device__ inline void oc_advance(Scalars* sc) {
   if (sc->phase == 1) oc_finish(sc);
 }
 
-// After the last step: measure what the choice still needs, or choose.
+// After the last step: measure what the choice still needs, or choose.  The
+// errors are exact where measured, lower bounds where implied (above:
+// V >= the bound, below: V <= the bound).  A bound is measured only when it
+// could change the decision: first between the final lo and hi, then the
+// winner's predecessor, which must have a strictly larger error.
 __host__ __device__ inline void oc_finish(Scalars* sc) {
   if (!sc->spec) {   // every midpoint was measured: the running choice stands
     sc->oc_done = 1;
     sc->lam_prev = sc->best_lam;
     return;
   }
-  int* st = sc->slot_state;
+  const int* st = sc->slot_state;
   const double* v = sc->slot_v;
-  int need = -1;
-  if (st[0] == kSlotBound) need = 0;
-  else if (st[2] == kSlotBound) need = 2;
-  else if (st[1] == kSlotBound && st[0] == kSlotExact && !(v[1] > v[0])) need = 1;
-  else if (st[3] == kSlotBound && st[2] == kSlotExact && !(v[3] < v[2])) need = 3;
-  if (need >= 0) {
-    sc->check = need;
-    sc->lam = sc->slot_lam[need];
-    return;
+  const double t = sc->target;
+  auto measure = [&](int k) {
+    sc->check = k;
+    sc->lam = sc->slot_lam[k];
+  };
+  const bool ha = st[0] != kSlotNone, hb = st[2] != kSlotNone;
+  const double ea = ha ? v[0] - t : INFINITY;   // exact, or a lower bound
+  const double eb = hb ? t - v[2] : INFINITY;
+  int win;   // 0: final lo, 2: final hi
+  if (ha && hb) {
+    const bool xa = st[0] == kSlotExact, xb = st[2] == kSlotExact;
+    if (xa && xb) {
+      if (ea == eb) {   // an exact tie across the sides: settle it by measuring all
+        sc->redo = 1;
+        sc->oc_done = 1;
+        return;
+      }
+      win = ea < eb ? 0 : 2;
+    } else if (xa && eb > ea) {
+      win = 0;
+    } else if (xb && ea > eb) {
+      win = 2;
+    } else {
+      // measure the bound that may be the smaller error
+      measure(!xa && (xb || ea <= eb) ? 0 : 2);
+      return;
+    }
+  } else {
+    win = ha ? 0 : 2;
+    if (st[win] == kSlotBound) {
+      measure(win);
+      return;
+    }
   }
-  // a predecessor with the same volume: the kept midpoint may lie further back
-  if ((st[0] == kSlotExact && st[1] == kSlotExact && v[1] == v[0]) ||
-      (st[2] == kSlotExact && st[3] == kSlotExact && v[3] == v[2])) {
+  // the winner's predecessor on its side: strictly larger error
+  const int p = win + 1;
+  if (st[p] == kSlotBound) {
+    const bool separated = (win == 0) ? (v[p] > v[0]) : (v[p] < v[2]);
+    if (!separated) {
+      measure(p);
+      return;
+    }
+  } else if (st[p] == kSlotExact && v[p] == v[win]) {
+    // a plateau: the kept midpoint may lie further back
     sc->redo = 1;
     sc->oc_done = 1;
     return;
   }
-  const bool ha = st[0] == kSlotExact, hb = st[2] == kSlotExact;
-  const double ea = ha ? v[0] - sc->target : INFINITY;
-  const double eb = hb ? sc->target - v[2] : INFINITY;
-  const bool take_a = ha && (!hb || ea < eb || (ea == eb && sc->slot_step[0] < sc->slot_step[2]));
-  if (take_a || hb) {
-    sc->best_lam = take_a ? sc->slot_lam[0] : sc->slot_lam[2];
-    sc->best_err = take_a ? ea : eb;
-  }
+  sc->best_lam = sc->slot_lam[win];
+  sc->best_err = (win == 0) ? ea : eb;
   sc->oc_done = 1;
   sc->lam_prev = sc->best_lam;
+  if (sc->ka > 0.0 && sc->kb < INFINITY && sc->kb > sc->ka)
+    sc->slope_prev = (sc->kva - sc->kvb) / log(sc->kb / sc->ka);
 }
 
 // Record an evaluation V at sc->lam and place the next one.
@@ -227,6 +260,13 @@ __host__ __device__ inline void oc_next_probe(Scalars* sc) {
       } else if (sc->nspec == 0 && sc->lam_prev > a && sc->lam_prev < b) {
         est = sc->lam_prev;                     // the previous design's multiplier
         sc->gfac = 1.05;
+      } else if (sc->nspec == 1 && sc->slope_prev > 0.0 && (have_a != have_b)) {
+        // one side known, from the previous multiplier: a Newton step in
+        // log lambda with the previous slope, overshot by a quarter so that it
+        // lands past the crossing
+        const double l0 = have_a ? sc->ka : sc->kb, v0 = have_a ? sc->kva : sc->kvb;
+        const double step = (v0 - sc->target) / sc->slope_prev;   // > 0 above, < 0 below
+        est = l0 * exp(1.25 * step);
       } else if (have_a) {
         est = sc->ka * sc->gfac;                // search upwards, doubling in log lambda
         sc->gfac *= sc->gfac;

@@ -55,6 +55,7 @@ struct Scalars {
   double lo, hi, lam, best_lam, best_err, target, V, ratio_scale, tol;
   double ka, kva, kb, kvb;   // evaluated multipliers nearest the crossing: V(ka) > target >= V(kb)
   double lam_prev;           // the previous bisection's kept multiplier (first probe)
+  double slope_prev;         // its -dV/dlog(lambda) near the crossing (0: unknown)
   double gfac;               // step factor of the one-sided search
   double slot_lam[4], slot_v[4];   // last two midpoints on each side of the target
   int slot_state[4], slot_step[4];
