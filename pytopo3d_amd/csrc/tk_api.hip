@@ -68,6 +68,7 @@ static int dmalloc(T** p, size_t n) {
 }
 
 static void free_plan(Plan* P) {
+  if (P->cstream) (void)hipStreamSynchronize(P->cstream);   // a factorisation may be running
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(P->KE64); f(P->KE32); f(P->sten_off); f(P->sten_w); f(P->Hs); f(P->eflags); f(P->fcols);
   f(P->sec64_dev); f(P->sec32_dev);
@@ -90,6 +91,9 @@ static void free_plan(Plan* P) {
   if (P->gev_in) (void)hipEventDestroy(P->gev_in);
   if (P->gev_out) (void)hipEventDestroy(P->gev_out);
   if (P->gstream) (void)hipStreamDestroy(P->gstream);
+  if (P->cstream) (void)hipStreamDestroy(P->cstream);
+  if (P->cev_in) (void)hipEventDestroy(P->cev_in);
+  if (P->cev_out) (void)hipEventDestroy(P->cev_out);
   delete P;
 }
 
@@ -247,6 +251,13 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   if (hipHostMalloc((void**)&P->sc_host, sizeof(Scalars)) != hipSuccess) {
     free_plan(P);
     set_error("hipHostMalloc failed");
+    return 2;
+  }
+  if (hipStreamCreateWithFlags(&P->cstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P->cev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->cev_out, hipEventDisableTiming) != hipSuccess) {
+    free_plan(P);
+    set_error("could not create the set-up stream");
     return 2;
   }
   std::vector<float> ke32(576);

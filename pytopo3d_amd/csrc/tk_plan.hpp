@@ -106,6 +106,11 @@ struct Plan {
   // once per plan and replayed on gstream.
   hipStream_t gstream = nullptr;
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
+  // the coarsest factorisation runs on cstream beside the rest of the set-up;
+  // the first coarsest solve after it waits for cev_out (coarse_pending)
+  hipStream_t cstream = nullptr;
+  hipEvent_t cev_in = nullptr, cev_out = nullptr;
+  bool coarse_pending = false;
   static constexpr int kChunkSizes = 4;   // chunk lengths 2 * base, base, base / 2, 2
   hipGraphExec_t gexec[2][2][kChunkSizes] = {};
   Counters gdelta[2][2][kChunkSizes];      // host counters added by one replay

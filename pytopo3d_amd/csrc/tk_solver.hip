@@ -266,6 +266,15 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
   return x;
 }
 
+// Make s wait for the coarsest factorisation of the last set-up, once.  The
+// first coarsest solve after a set-up is always launched eagerly (before any
+// Krylov chunk is captured), so no captured graph holds this wait.
+static void join_coarse(Plan& P, hipStream_t s) {
+  if (!P.coarse_pending) return;
+  (void)hipStreamWaitEvent(s, P.cev_out, 0);
+  P.coarse_pending = false;
+}
+
 // z = V-cycle(b) on the hierarchy; returns the pointer holding z (a level-0 buffer).
 template <typename T>
 static T* vcycle(Plan& P, const T* b0, hipStream_t s) {
@@ -287,6 +296,7 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s) {
   }
   {
     LevelBuf<T>& C = buf<T>(P.lv[nl - 1]);
+    join_coarse(P, s);
     launch_coarse_solve<T>(P, rhs[nl - 1], C.x, s);
     cur[nl - 1] = C.x;
   }
@@ -308,6 +318,7 @@ template <typename T>
 static const T* precondition(Plan& P, const T* r, T* zbuf, int jacobi, hipStream_t s) {
   if (jacobi || P.nlev == 1) {
     if (P.nlev == 1 && !jacobi) {
+      join_coarse(P, s);
       launch_coarse_solve<T>(P, r, zbuf, s);
       return zbuf;
     }
@@ -536,12 +547,21 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
     if (prec != 0)
       launch_coarsen_scale<float>(P.lv[l], P.lv[l + 1], P.lv[l].f32.s, P.lv[l + 1].f32.s, s);
   }
+  // The coarsest factorisation (one workgroup, ~0.4 ms) runs on the plan's
+  // set-up stream beside the inverse diagonals, the right-hand side, the
+  // initial residual and the first V-cycle's descent; the first coarsest solve
+  // waits for it (join_coarse).  The set-up stream starts after everything
+  // queued on s so far, so it never overwrites a factor still in use.
+  TK_HIP(hipEventRecord(P.cev_in, s));
+  TK_HIP(hipStreamWaitEvent(P.cstream, P.cev_in, 0));
+  launch_coarse_setup(P, P.cstream);
+  TK_HIP(hipEventRecord(P.cev_out, P.cstream));
+  P.coarse_pending = true;
   for (int l = 0; l < P.nlev; ++l) {
     Level& L = P.lv[l];
     launch_diag_inv<double>(P, L, L.f64.s, L.f64.dinv, (double*)nullptr, s);
     if (prec != 0) launch_diag_inv<float>(P, L, L.f32.s, L.f32.dinv, (float*)nullptr, s);
   }
-  launch_coarse_setup(P, s);
   TK_HIP(hipGetLastError());
   P.last_penal = penal;
   P.last_prec = prec;
