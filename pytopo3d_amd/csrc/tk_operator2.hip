@@ -31,6 +31,8 @@
 #include "tk_plan.hpp"
 #include "tk_sector.hpp"
 
+#include <type_traits>
+
 namespace tk {
 
 template <typename T>
@@ -78,13 +80,26 @@ constexpr int kTile = 16;          // threads per tile edge
 constexpr int kOut = kTile - 1;    // owned nodes per tile edge
 constexpr int kPl = kTile + 1;     // node-plane edge held in LDS
 
+// Registers of one node plane.  `fetch` only issues loads into them; the
+// operand is formed and masked in `store`, a layer later, so no wave waits on
+// loads that were just issued (the wait counter is in order: one early wait
+// would also wait for every load issued before it).
 template <typename T, int MODE>
 struct PlaneRegs {
-  T own[3];      // own node operand, unmasked
+  // kModeApplyDir: the direction is an FP32 value (stored in FP32), so it is
+  // held in FP32 and widened where it enters the FP64 operator
+  using OwnT = std::conditional_t<MODE == kModeApplyDir, float, T>;
+  OwnT own[3];   // own node operand, unmasked (formed in `store`; kModeChebZero
+                 // keeps b and D^-1 instead and forms it again in the epilogue)
   uint8_t fix;   // own node fixed bits (7 outside the domain)
-  T ext[3];      // extra halo node operand, masked
+  T ext[3];      // extra halo node operand (kModeChebFirstP: formed and masked)
+  uint8_t xfix;  // extra node fixed bits (7 outside the domain)
   T ob[3];       // kModeChebZero: own b and D^-1 (reused by the epilogue)
   T odi[3];
+  T xb[3];       // kModeChebZero: extra node b and D^-1
+  T xdi[3];
+  float oz[3], op[3];   // kModeApplyDir: own z32 and previous direction
+  float xz[3], xp[3];   // ... of the extra node
 };
 
 // Loads of element layer ez: its element scale and the epilogue operands of
@@ -147,9 +162,11 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     dir_zs = (float)ep.sc->zs;
     dir_beta = ep.p32o ? (float)ep.sc->beta : 0.0f;
   }
-  auto dir_value = [&](int n, int c) -> T {
-    return (T)(dir_zs * ep.z32[3 * n + c] + dir_beta * dir_p[3 * n + c]);
-  };
+  // one explicit rounding order: the owner and the neighbouring tiles (halo
+  // node) must form bitwise the same direction, or K p no longer matches the
+  // stored p and the FP64 residual drifts from the true one; left to
+  // contraction, the two call sites were fused differently
+  auto dir_value = [&](float z, float p) -> float { return __builtin_fmaf(dir_zs, z, dir_beta * p); };
   // kModeChebFirstP: (P xc) at fine node (iy, ix, iz), trilinear weights (an
   // even fine index has one coarse parent per axis: weight 1, duplicate 0)
   auto prol = [&](int iy, int ix, int iz, T out[3]) {
@@ -171,10 +188,14 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
       out[2] += w * ep.xc[3 * I + 2];
     }
   };
-  // global loads of node plane pz (registers only)
+  // global loads of node plane pz (registers only; see PlaneRegs)
   auto fetch = [&](int pz, PlaneRegs<T, MODE>& R) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) R.own[c] = R.ob[c] = R.odi[c] = T(0);
+    for (int c = 0; c < 3; ++c) {
+      R.own[c] = 0;
+      R.ob[c] = R.odi[c] = T(0);
+      if constexpr (MODE == kModeApplyDir) R.oz[c] = R.op[c] = 0.0f;
+    }
     R.fix = 7;
     if (col_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = ey + ex * g.nny + pz * nxy;
@@ -186,9 +207,9 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         if constexpr (MODE == kModeChebZero) {
           R.ob[c] = ep.b[3 * n + c];
           R.odi[c] = ep.dinv[3 * n + c];
-          R.own[c] = ep.c0 * R.odi[c] * R.ob[c];
         } else if constexpr (MODE == kModeApplyDir) {
-          R.own[c] = dir_value(n, c);
+          R.oz[c] = ep.z32[3 * n + c];
+          R.op[c] = dir_p[3 * n + c];
         } else if constexpr (MODE == kModeChebFirstP) {
           R.own[c] = u[3 * n + c] + (((R.fix >> c) & 1) ? T(0) : pc[c]);
         } else {
@@ -196,29 +217,60 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         }
       }
     }
-    R.ext[0] = R.ext[1] = R.ext[2] = T(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R.ext[c] = R.xb[c] = R.xdi[c] = T(0);
+      if constexpr (MODE == kModeApplyDir) R.xz[c] = R.xp[c] = 0.0f;
+    }
+    R.xfix = 7;
     if (ext_node && (unsigned)pz < (unsigned)g.nnz) {
       const int n = xny + xnx * g.nny + pz * nxy;
-      const uint8_t f = fixed[n];
-      T pc[3] = {T(0), T(0), T(0)};
-      if constexpr (MODE == kModeChebFirstP) prol(xny, xnx, pz, pc);
+      R.xfix = fixed[n];
+      if constexpr (MODE == kModeChebFirstP) {
+        // the gathered coarse values are summed at once (24 more registers
+        // otherwise); this sweep runs on the small levels only
+        T pc[3];
+        prol(xny, xnx, pz, pc);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T v;
-        if constexpr (MODE == kModeChebZero) v = ep.c0 * ep.dinv[3 * n + c] * ep.b[3 * n + c];
-        else if constexpr (MODE == kModeApplyDir) v = dir_value(n, c);
-        else if constexpr (MODE == kModeChebFirstP) v = u[3 * n + c] + pc[c];
-        else v = u[3 * n + c];
-        R.ext[c] = ((f >> c) & 1) ? T(0) : v;
+        for (int c = 0; c < 3; ++c) R.ext[c] = u[3 * n + c] + pc[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if constexpr (MODE == kModeChebZero) {
+            R.xb[c] = ep.b[3 * n + c];
+            R.xdi[c] = ep.dinv[3 * n + c];
+          } else if constexpr (MODE == kModeApplyDir) {
+            R.xz[c] = ep.z32[3 * n + c];
+            R.xp[c] = dir_p[3 * n + c];
+          } else {
+            R.ext[c] = u[3 * n + c];
+          }
+        }
       }
     }
   };
-  auto store = [&](int buf, const PlaneRegs<T, MODE>& R) {
+  // forms the plane's operands (kept in R.own for the epilogue) and writes the
+  // masked values to LDS
+  auto store = [&](int buf, PlaneRegs<T, MODE>& R) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) un[buf][c][ly + kPl * lx] = ((R.fix >> c) & 1) ? T(0) : R.own[c];
+    for (int c = 0; c < 3; ++c) {
+      T v;
+      if constexpr (MODE == kModeChebZero) {
+        v = ep.c0 * R.odi[c] * R.ob[c];
+      } else {
+        if constexpr (MODE == kModeApplyDir) R.own[c] = dir_value(R.oz[c], R.op[c]);
+        v = (T)R.own[c];
+      }
+      un[buf][c][ly + kPl * lx] = ((R.fix >> c) & 1) ? T(0) : v;
+    }
     if (xi >= 0)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) un[buf][c][xi + kPl * xj] = R.ext[c];
+      for (int c = 0; c < 3; ++c) {
+        T v = R.ext[c];
+        if constexpr (MODE == kModeChebZero) v = ep.c0 * R.xdi[c] * R.xb[c];
+        if constexpr (MODE == kModeApplyDir) v = (T)dir_value(R.xz[c], R.xp[c]);
+        un[buf][c][xi + kPl * xj] = ((R.xfix >> c) & 1) ? T(0) : v;
+      }
   };
   // face values of node plane buf at this thread's element column, then the
   // in-plane transform
@@ -268,7 +320,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   store(0, RB);
   store(1, RA);
   // own node of the plane being finished (the bottom plane of the layer)
-  T raw_prev[3] = {RB.own[0], RB.own[1], RB.own[2]};
+  typename PlaneRegs<T, MODE>::OwnT raw_prev[3] = {RB.own[0], RB.own[1], RB.own[2]};
   T pb_prev[3] = {RB.ob[0], RB.ob[1], RB.ob[2]};
   T pdi_prev[3] = {RB.odi[0], RB.odi[1], RB.odi[2]};
   uint8_t fix_prev = RB.fix;
@@ -358,10 +410,12 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         const T ku = nb[c][0] + yo[1][c][tym] + yo[0][c][txm] + yo[2][c][tbm];
         const int i = 3 * n_out + c;
         const bool fx = (fix_prev >> c) & 1;
-        const T val = fx ? raw_prev[c] : ku;
+        // the node operand (kModeChebZero: x1 = c0 D^-1 b, formed again as in `store`)
+        const T xn = (MODE == kModeChebZero) ? ep.c0 * edi[c] * eb[c] : (T)raw_prev[c];
+        const T val = fx ? xn : ku;
         if (MODE == kModeApply || MODE == kModeApplyDir) {
           ep.y[i] = val;
-          dot += (double)raw_prev[c] * (double)val;
+          dot += (double)xn * (double)val;
           if (MODE == kModeApplyDir) ep.p32n[i] = (float)raw_prev[c];
         } else if (MODE == kModeResidual) {
           // zero on fixed DOFs: the restriction reads no fine mask
@@ -369,26 +423,30 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         } else if (MODE == kModeChebFirst || MODE == kModeChebFirstP) {
           const T dd = ep.c0 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
-          ep.xo[i] = raw_prev[c] + dd;
+          ep.xo[i] = xn + dd;
         } else if (MODE == kModeChebStep) {
           const T dd = ep.c1 * ed[c] + ep.c2 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
-          ep.xo[i] = raw_prev[c] + dd;
-        } else {   // kModeChebZero: d1 = x1 = raw_prev
-          const T dd = ep.c1 * raw_prev[c] + ep.c2 * edi[c] * (eb[c] - val);
+          ep.xo[i] = xn + dd;
+        } else {   // kModeChebZero: d1 = x1
+          const T dd = ep.c1 * xn + ep.c2 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
-          ep.xo[i] = raw_prev[c] + dd;
+          ep.xo[i] = xn + dd;
         }
       }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      raw_prev[c] = nxt.own[c];
+      if constexpr (MODE != kModeChebZero) raw_prev[c] = nxt.own[c];
       pb_prev[c] = nxt.ob[c];
       pdi_prev[c] = nxt.odi[c];
     }
     fix_prev = nxt.fix;
-    if (more) store(buf ^ 1, pre);
+    // unconditional: after the last layer `pre` still holds an earlier plane
+    // and the LDS buffer is not read again; a conditional store would leave a
+    // path without the load wait, and the compiler then waits for all loads
+    // in flight before the next layer's first use of a prefetched value
+    store(buf ^ 1, pre);
     __syncthreads();
   };
 

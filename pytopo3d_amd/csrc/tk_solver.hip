@@ -194,9 +194,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir_flex(int n, const float* __re
   if (*done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  // formed in FP32, as the fused operator sweep does
-  float pf = (float)sc->zs * z32[i];
-  if (p32o) pf += (float)sc->beta * p32o[i];
+  // formed in FP32 with the fused operator sweep's rounding order
+  const float pb = p32o ? (float)sc->beta * p32o[i] : 0.0f;
+  const float pf = __builtin_fmaf((float)sc->zs, z32[i], pb);
   p32n[i] = pf;
   p64[i] = (double)pf;
 }

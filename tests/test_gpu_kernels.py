@@ -173,6 +173,33 @@ def test_solve_matches_direct_solve(nelx, nely, nelz, precision, solver, kind):
     P.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_true_residual_across_tiles(precision):
+    """The recursive FP64 residual of the Krylov loop must stay the true one:
+    on a grid of 3 x 3 element-kernel tiles in x-y and several z chunks, the
+    residual F - K U recomputed with the FP64 operator after the solve is at
+    the requested tolerance.  (A direction formed with a different rounding in
+    the owning and the neighbouring tile makes K p differ from the stored p;
+    the true residual then stalls orders of magnitude above the recursive
+    one.)"""
+    nelx, nely, nelz = 40, 34, 20
+    note("true residual", nelx, nely, nelz, precision)
+    rng = np.random.default_rng(23)
+    P, _ = make_plan(nelx, nely, nelz, 1.5)
+    rd = to_dev(elem_vec(rng.uniform(0.01, 1.0, (nely, nelx, nelz))))
+    Fd = to_dev(oracle.edge_load_force_vector(nelx, nely, nelz))
+    U = torch.zeros_like(Fd)
+    rtol = 1e-11
+    it, rr = P.solve(rd, 3.0, Fd, U, rtol=rtol, maxit=20000, precision=precision)
+    y = torch.empty_like(U)
+    P.apply_K(rd, 3.0, U, y)
+    true_rel = float(torch.linalg.vector_norm(Fd - y) / torch.linalg.vector_norm(Fd))
+    note("  iters", it, "recursive", rr, "true", true_rel)
+    assert rr <= rtol
+    assert true_rel <= 10 * rtol, true_rel
+    P.close()
+
+
 @pytest.mark.parametrize("nelx,nely,nelz,rmin", [(6, 4, 3, 1.5), (10, 6, 4, 2.5)])
 def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin):
     note("energy", nelx, nely, nelz, rmin)
