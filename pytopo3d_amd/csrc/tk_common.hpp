@@ -70,6 +70,7 @@ struct Scalars {
   double sum0, sum1;   // scratch sums
   double norm0, norm1; // scratch norms
   double zs;           // scale of the FP32 preconditioner input (flexible CG)
+  double qz;           // q.z of the last flexible-CG update (next beta)
 };
 
 // ---------------------------------------------------------------------------

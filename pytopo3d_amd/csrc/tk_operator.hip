@@ -487,14 +487,16 @@ void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, cons
 
 template <typename T>
 void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x,
-                      T* xo, T* d, double c1, double c2, int write_d, hipStream_t s) {
+                      T* xo, T* d, double c1, double c2, int write_d, hipStream_t s,
+                      int* dot_blocks) {
   if (&L == &P.lv[0]) {
     if (sizeof(T) == 8) P.cnt.applies64++; else P.cnt.applies32++;
   }
   if (P.use_el) {
-    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, 0.0, c1, c2, 0, write_d, done_ptr(P), s);
+    el_cheb<T>(P, L, sl, dinv, b, x, xo, d, 0.0, c1, c2, 0, write_d, done_ptr(P), s, dot_blocks);
     return;
   }
+  if (dot_blocks) *dot_blocks = 0;   // the node kernel forms no partials
   hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblocks(L.g.nnode)), dim3(kBlock), 0, s, L.g,
                      ke_of<T>(P), sl, L.fixed, dinv, b, x, xo, d, (T)c1, (T)c2, done_ptr(P));
 }
@@ -550,7 +552,7 @@ hipError_t coarse_setup_attr(size_t lds) {
   template void launch_cheb_first<T>(Plan&, const Level&, const T*, const T*, const T*, const T*,  \
                                      T*, T*, double, int, int, hipStream_t);                       \
   template void launch_cheb_step<T>(Plan&, const Level&, const T*, const T*, const T*, const T*,   \
-                                    T*, T*, double, double, int, hipStream_t);                     \
+                                    T*, T*, double, double, int, hipStream_t, int*);               \
   template void launch_restrict<T>(const Level&, const Level&, const T*, T*, hipStream_t,          \
                                    const int*);                                                    \
   template void launch_prolong_add<T>(const Level&, const Level&, const T*, T*, hipStream_t,       \

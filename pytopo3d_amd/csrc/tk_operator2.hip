@@ -427,7 +427,9 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         } else if (MODE == kModeChebStep) {
           const T dd = ep.c1 * ed[c] + ep.c2 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
-          ep.xo[i] = xn + dd;
+          const T xnew = xn + dd;
+          ep.xo[i] = xnew;
+          if (with_dot) dot += (double)eb[c] * (double)xnew;   // r.z of the Krylov loop
         } else {   // kModeChebZero: d1 = x1
           const T dd = ep.c1 * xn + ep.c2 * edi[c] * (eb[c] - val);
           if (ep.d) ep.d[i] = dd;
@@ -455,7 +457,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     if (ez + 1 >= Z1) break;
     layer(ez + 1, 0, F1, F0, RB, RA, LA, LB);
   }
-  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot) {
+  if ((MODE == kModeApply || MODE == kModeApplyDir || MODE == kModeChebStep) && with_dot) {
     double v[1] = {dot};
     block_partials<1>(v, partials);
   }
@@ -604,9 +606,10 @@ static int el_resident() {
   return v;
 }
 
+// Returns the number of workgroups (= partials per reduced quantity).
 template <typename T, int MODE>
-static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
-                      int with_dot, const int* done, hipStream_t s) {
+static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
+                     int with_dot, const int* done, hipStream_t s) {
   int nty, zc;
   dim3 grid;
   // the reduction scratch holds 4 values per block of the largest 1-D launch
@@ -623,6 +626,7 @@ static void launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Ep
                        P.partials);
   if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot)
     finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
+  return (int)(grid.x * grid.y);
 }
 
 // Flexible-CG direction and operator in one sweep (see kModeApplyDir):
@@ -658,10 +662,12 @@ void el_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T
   launch_el<T, kModeResidual>(P, L, sl, x, ep, 0, done, s);
 }
 
+// dot_blocks (Chebyshev step only): the sweep also leaves the partials of
+// b.xo, the Krylov r.z when this is the last step of the V-cycle.
 template <typename T>
 void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x, T* xo,
              T* d, double c0, double c1, double c2, int first, int write_d, const int* done,
-             hipStream_t s) {
+             hipStream_t s, int* dot_blocks) {
   Epilogue<T> ep{};
   ep.b = b;
   ep.xo = xo;
@@ -671,8 +677,13 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   ep.c0 = (T)c0;
   ep.c1 = (T)c1;
   ep.c2 = (T)c2;
-  if (first) launch_el<T, kModeChebFirst>(P, L, sl, x, ep, 0, done, s);
-  else launch_el<T, kModeChebStep>(P, L, sl, x, ep, 0, done, s);
+  if (first) {
+    launch_el<T, kModeChebFirst>(P, L, sl, x, ep, 0, done, s);
+    if (dot_blocks) *dot_blocks = 0;
+  } else {
+    const int G = launch_el<T, kModeChebStep>(P, L, sl, x, ep, dot_blocks ? 1 : 0, done, s);
+    if (dot_blocks) *dot_blocks = G;
+  }
 }
 
 // First post-smoothing step on x + P xc (see kModeChebFirstP).
@@ -713,7 +724,7 @@ void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T*
   template void el_residual<T>(Plan&, const Level&, const T*, const T*, const T*, T*,           \
                                const int*, hipStream_t);                                        \
   template void el_cheb<T>(Plan&, const Level&, const T*, const T*, const T*, const T*, T*, T*, \
-                           double, double, double, int, int, const int*, hipStream_t);  \
+                           double, double, double, int, int, const int*, hipStream_t, int*); \
   template void el_cheb_zero2<T>(Plan&, const Level&, const T*, const T*, const T*, T*, T*,     \
                                  double, double, double, int, const int*, hipStream_t);         \
   template void el_cheb_prolong<T>(Plan&, const Level&, const Level&, const T*, const T*,       \

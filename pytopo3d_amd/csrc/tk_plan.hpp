@@ -120,8 +120,8 @@ struct Plan {
 // ---- two-phase reductions (tk_reduce.hip) -----------------------------------
 enum FinOp {
   kFinSum0 = 0, kFinSum1, kFinNorm0, kFinNorm1, kFinC,
-  kFinAlpha, kFinNorms, kFinUpdate, kFinRz, kFinRzFlex,
-  kFinRatioMax, kFinOcVolume, kFinStats
+  kFinAlpha, kFinNorms, kFinUpdate, kFinRz,
+  kFinRatioMax, kFinOcVolume, kFinStats, kFinUpdateFlex, kFinRzPR
 };
 constexpr int kRedBlocks = 2048;   // grid-stride reduction kernels use at most this many blocks
 void finalize(Plan& P, int op, int G, hipStream_t s, double a0 = 0.0, int i0 = 0);
@@ -160,9 +160,13 @@ void launch_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* 
 template <typename T>
 void launch_cheb_first(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
                        const T* x, T* xo, T* d, double c0, int x_zero, int write_d, hipStream_t s);
+// dot_blocks != nullptr asks the step for the partials of b.xo (one per
+// workgroup, in the plan's reduction scratch); it receives their count, or 0
+// when this launch cannot form them (the caller then reduces separately).
 template <typename T>
 void launch_cheb_step(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b,
-                      const T* x, T* xo, T* d, double c1, double c2, int write_d, hipStream_t s);
+                      const T* x, T* xo, T* d, double c1, double c2, int write_d, hipStream_t s,
+                      int* dot_blocks = nullptr);
 template <typename T>
 void launch_restrict(const Level& f, const Level& c, const T* rf, T* bc, hipStream_t s,
                      const int* done);
@@ -188,7 +192,7 @@ void el_residual(Plan& P, const Level& L, const T* sl, const T* b, const T* x, T
 template <typename T>
 void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, const T* x, T* xo,
              T* d, double c0, double c1, double c2, int first, int write_d, const int* done,
-             hipStream_t s);
+             hipStream_t s, int* dot_blocks = nullptr);
 template <typename T>
 void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, T* xo, T* d,
                    double c0, double c1, double c2, int write_d, const int* done, hipStream_t s);

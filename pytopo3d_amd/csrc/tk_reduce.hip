@@ -11,11 +11,13 @@ namespace tk {
 
 __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* __restrict__ partials,
                                                   Scalars* sc, double a0, int i0) {
-  if ((op == kFinAlpha || op == kFinUpdate || op == kFinRz || op == kFinRzFlex) && sc->done) return;
+  if ((op == kFinAlpha || op == kFinUpdate || op == kFinRz || op == kFinUpdateFlex ||
+       op == kFinRzPR) && sc->done)
+    return;
   if (op == kFinOcVolume && sc->oc_done) return;
   double s[3] = {0.0, 0.0, 0.0};
   double m = -INFINITY;
-  const bool two = (op == kFinNorms || op == kFinRzFlex || op == kFinStats);
+  const bool two = (op == kFinNorms || op == kFinStats || op == kFinUpdateFlex);
   if (op == kFinRatioMax) {
     for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmax(m, partials[b]);
   } else {
@@ -70,11 +72,24 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
       sc->rz = s[0];
       if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
       break;
-    case kFinRzFlex:  // s0 = r.z, s1 = z.K p_old: beta = -z.K p_old / p_old.K p_old
-      sc->beta = i0 ? 0.0 : -s[1] / sc->pq;
-      sc->rz = s[0];
-      if (!(s[0] > 0.0)) { sc->done = 1; sc->stalled = (s[0] == 0.0) ? 0 : 1; }
+    case kFinUpdateFlex:   // kFinUpdate, and s1 = q.z for the next beta
+      sc->zs = sqrt(sc->rr);
+      sc->rr = s[0];
+      sc->qz = s[1];
+      sc->iter += 1;
+      if (s[0] <= sc->tol2 * sc->bb) sc->done = 1;
+      if (sc->iter >= sc->maxit) sc->done = 1;
+      if (!(s[0] == s[0])) { sc->done = 1; sc->stalled = 1; }
       break;
+    case kFinRzPR: {   // s0 = r.z (i0 bit 1: of the scaled vectors, times zs^2); i0 bit 0: first
+      const double rz = (i0 & 2) ? s[0] * sc->zs * sc->zs : s[0];
+      // flexible (Polak-Ribiere) beta: (r.z - r.z_old) / rz_old, where
+      // r.z_old = rz_old - alpha q_old.z_old
+      sc->beta = (i0 & 1) ? 0.0 : (rz - (sc->rz - sc->alpha * sc->qz)) / sc->rz;
+      sc->rz = rz;
+      if (!(rz > 0.0)) { sc->done = 1; sc->stalled = (rz == 0.0) ? 0 : 1; }
+      break;
+    }
     case kFinRatioMax: sc->ratio_scale = m; break;
     case kFinOcVolume:   // the candidate was formed at sc->lam
       oc_record(sc, s[0]);

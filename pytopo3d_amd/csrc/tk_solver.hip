@@ -141,32 +141,35 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __r
 }
 
 // x += alpha p (p stored in FP32); r -= alpha q; r32 = r / ||r_old|| (the norm
-// before this update; kFinUpdate records it as zs); r.r partials.
+// before this update; kFinUpdateFlex records it as zs); partials of r.r and of
+// q.z with z = zs z32 the correction p was formed from (the next beta).
 __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __restrict__ x,
                                                             double* __restrict__ r,
                                                             const float* __restrict__ p32,
                                                             const double* __restrict__ q,
                                                             float* __restrict__ r32,
+                                                            const float* __restrict__ z32,
                                                             const int* __restrict__ done,
                                                             double* partials, const Scalars* sc) {
   if (*done) return;
   const double alpha = sc->alpha;
   const double rr = sc->rr;
+  const double zs = sc->zs;
   const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
-  double v[1] = {0.0};
+  double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] += alpha * (double)p32[i];
-    const double ri = r[i] - alpha * q[i];
+    const double qi = q[i];
+    const double ri = r[i] - alpha * qi;
     r[i] = ri;
     r32[i] = (float)(ri * inv);
     v[0] += ri * ri;
+    v[1] += qi * (zs * (double)z32[i]);
   }
-  block_partials<1>(v, partials);
+  block_partials<2>(v, partials);
 }
 
-// z = zs z32; partials of r.z and q.z, q = K p_old (kFinRzFlex:
-// beta = -(z.K p_old) / (p_old.K p_old), the flexible-CG orthogonalisation
-// against the previous direction)
+// z = zs z32; partials of r.z and (unless first) q.z, q = K p_old
 __global__ __launch_bounds__(kBlock) void k_cg_rz_flex(int n, const double* __restrict__ r,
                                                         const double* __restrict__ q,
                                                         const float* __restrict__ z32,
@@ -218,9 +221,11 @@ constexpr int kProlongFuseMaxNodes = 100000;
 
 // xc != nullptr (post-smoothing, element kernel): the smoothing starts from
 // x + P xc, the coarse-grid correction being prolongated inside the first step.
+// dot_blocks != nullptr: the last step also leaves the partials of b.x (see
+// launch_cheb_step); their count, 0 if it could not.
 template <typename T>
 static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s,
-                 const T* xc = nullptr) {
+                 const T* xc = nullptr, int* dot_blocks = nullptr) {
   Level& L = P.lv[l];
   // the coarse levels are latency-bound: a longer smoother over a wider
   // interval costs little there and strengthens the cycle
@@ -256,10 +261,12 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
                          deg > 1, s);
     std::swap(x, xa);
   }
+  if (dot_blocks) *dot_blocks = 0;
   for (; k < deg; ++k) {
     const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+    const bool last = k + 1 == deg;
     launch_cheb_step<T>(P, L, B.s, B.dinv, b, x, xa, B.d, rho_new * rho_old, 2.0 * rho_new / delta,
-                        k + 1 < deg, s);
+                        !last, s, last ? dot_blocks : nullptr);
     std::swap(x, xa);
     rho_old = rho_new;
   }
@@ -275,9 +282,10 @@ static void join_coarse(Plan& P, hipStream_t s) {
   P.coarse_pending = false;
 }
 
-// z = V-cycle(b) on the hierarchy; returns the pointer holding z (a level-0 buffer).
+// z = V-cycle(b) on the hierarchy; returns the pointer holding z (a level-0
+// buffer).  dot_blocks: as for smooth, for the last fine-level step (b.z).
 template <typename T>
-static T* vcycle(Plan& P, const T* b0, hipStream_t s) {
+static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
   const int nl = P.nlev;
   const int* done = &P.sc->done;
   T* cur[kMaxLevels];
@@ -304,18 +312,23 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s) {
     // small levels: the prolongation rides in the first smoothing sweep (one
     // launch less); large levels: the separate sweep is faster than gathering
     // the coarse correction inside the register-heavy smoothing kernel
+    int* db = l == 0 ? dot_blocks : nullptr;
     if (P.use_el && P.lv[l].g.nnode <= kProlongFuseMaxNodes) {
-      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s, cur[l + 1]);
+      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s, cur[l + 1], db);
     } else {
       launch_prolong_add<T>(P.lv[l + 1], P.lv[l], cur[l + 1], cur[l], s, done);
-      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s);
+      cur[l] = smooth<T>(P, l, rhs[l], cur[l], alt[l], 0, s, nullptr, db);
     }
   }
   return cur[0];
 }
 
+// dot_blocks: receives the count of r.z partials the preconditioner's last
+// sweep left (0: none; the caller reduces r.z itself).
 template <typename T>
-static const T* precondition(Plan& P, const T* r, T* zbuf, int jacobi, hipStream_t s) {
+static const T* precondition(Plan& P, const T* r, T* zbuf, int jacobi, hipStream_t s,
+                             int* dot_blocks) {
+  *dot_blocks = 0;
   if (jacobi || P.nlev == 1) {
     if (P.nlev == 1 && !jacobi) {
       join_coarse(P, s);
@@ -327,7 +340,7 @@ static const T* precondition(Plan& P, const T* r, T* zbuf, int jacobi, hipStream
                        r, zbuf, &P.sc->done);
     return zbuf;
   }
-  return vcycle<T>(P, r, s);
+  return vcycle<T>(P, r, s, dot_blocks);
 }
 
 // ------------------------------------------------------------------ chunk runner
@@ -451,10 +464,19 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   }
   hipLaunchKernelGGL(k_cg_norms<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, b, r, P.partials);
   finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
+  // r.z: from the V-cycle's last fine sweep when it formed the partials
+  auto reduce_rz = [&](const T* zz, int G, int first, hipStream_t st) {
+    if (G == 0) {
+      hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, zz, done,
+                         P.partials);
+      G = red_blocks(n);
+    }
+    finalize(P, kFinRz, G, st, 0.0, first);
+  };
   // z buffer: q is free until the first operator application
-  const T* z = precondition<T>(P, r, q, jacobi, s);
-  hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, r, z, done, P.partials);
-  finalize(P, kFinRz, red_blocks(n), s, 0.0, 1);
+  int G = 0;
+  const T* z = precondition<T>(P, r, q, jacobi, s, &G);
+  reduce_rz(z, G, 1, s);
   hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, p, 1, done, P.sc);
   auto body = [&](hipStream_t st, int m) {
     for (int k = 0; k < m; ++k) {
@@ -463,10 +485,9 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
                          done, P.partials, P.sc);
       finalize(P, kFinUpdate, red_blocks(n), st);
       // the preconditioner output must not alias q (read by nothing after the update)
-      const T* zz = precondition<T>(P, r, q, jacobi, st);
-      hipLaunchKernelGGL(k_cg_rz<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, zz, done,
-                         P.partials);
-      finalize(P, kFinRz, red_blocks(n), st, 0.0, 0);
+      int Gk = 0;
+      const T* zz = precondition<T>(P, r, q, jacobi, st, &Gk);
+      reduce_rz(zz, Gk, 0, st);
       hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, zz, p, 0, done,
                          P.sc);
     }
@@ -484,8 +505,13 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
 // applied to the scaled FP64 residual: every correction is computed in FP32,
 // the search direction is stored in FP32, while q = K p, the residual
 // recurrence, dot products and step lengths stay FP64, so the iteration
-// reaches FP64 accuracy whatever the stiffness contrast.  Directions are
-// orthogonalised against the previous one only (flexible CG with truncation 1).
+// reaches FP64 accuracy whatever the stiffness contrast.  The direction update
+// is the flexible (Polak-Ribiere) one,
+//   beta_k = (r_k.z_k - r_k.z_{k-1}) / (r_{k-1}.z_{k-1}),
+//   r_k.z_{k-1} = r_{k-1}.z_{k-1} - alpha_{k-1} q_{k-1}.z_{k-1},
+// so it needs r.z (left by the V-cycle's last fine sweep, from the FP32
+// residual it smooths against) and q.z (from the update pass): no pass of its
+// own over the FP64 vectors.
 static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rtol, int maxit,
                      int jacobi, hipStream_t s) {
   const int n = P.g.ndof;
@@ -501,10 +527,16 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
   finalize(P, kFinNorms, red_blocks(n), s, rtol * rtol, maxit);
   hipLaunchKernelGGL(k_to_f32_unit, dim3(nblocks(n)), dim3(kBlock), 0, s, n, r, r32, done, P.sc);
   auto correction = [&](int first, hipStream_t st) -> const float* {
-    const float* z = precondition<float>(P, r32, P.q32, jacobi, st);
-    hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, q, z, first,
-                       done, P.partials, P.sc);
-    finalize(P, kFinRzFlex, red_blocks(n), st, 0.0, first);
+    int G = 0;
+    const float* z = precondition<float>(P, r32, P.q32, jacobi, st, &G);
+    if (G > 0) {
+      // partials of r32.z32: r.z = zs^2 r32.z32
+      finalize(P, kFinRzPR, G, st, 0.0, first | 2);
+    } else {
+      hipLaunchKernelGGL(k_cg_rz_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, r, q, z, 1,
+                         done, P.partials, P.sc);
+      finalize(P, kFinRzPR, red_blocks(n), st, 0.0, first);
+    }
     return z;
   };
   const float* z32 = correction(1, s);
@@ -520,8 +552,8 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
       launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, st);
     }
     hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, pnew,
-                       q, r32, done, P.partials, P.sc);
-    finalize(P, kFinUpdate, red_blocks(n), st);
+                       q, r32, z32, done, P.partials, P.sc);
+    finalize(P, kFinUpdateFlex, red_blocks(n), st);
     z32 = correction(0, st);
   };
   iteration(pbuf[0], nullptr, s);
