@@ -536,7 +536,7 @@ int tk_smoother_step(tk_plan* plan, int prec, const void* b, const void* x, void
     set_error("unknown precision");
     return 1;
   }
-  if (P.last_penal < 0.0 || (prec == TK_MIXED_FP32 && P.last_prec == TK_FP64)) {
+  if (P.last_penal < 0.0 || prec != P.last_prec) {
     set_error("tk_smoother_step needs a previous tk_solve in this precision");
     return 1;
   }
@@ -594,8 +594,9 @@ int tk_solve(tk_plan* plan, int prec, const double* rho, double penal, const dou
     set_error("unknown precision");
     return 1;
   }
-  if ((flags & TK_SOLVE_REUSE) &&
-      (P.last_penal != penal || (prec == TK_MIXED_FP32 && P.last_prec == TK_FP64))) {
+  // each set-up builds the state of its own precision only (a mixed set-up
+  // forms no FP64 inverse diagonals)
+  if ((flags & TK_SOLVE_REUSE) && (P.last_penal != penal || prec != P.last_prec)) {
     set_error("TK_SOLVE_REUSE needs a previous solve or tk_prepare on this plan with the same rho, "
               "penal and precision");
     return 1;

@@ -235,6 +235,17 @@ __global__ __launch_bounds__(kBlock) void k_stiffness_scale(int n, const double*
   s[e] = (T)(kEmin + (1.0 - kEmin) * pow(rho[e], penal));
 }
 
+// The same in FP64 and its FP32 rounding in one pass.
+__global__ __launch_bounds__(kBlock) void k_stiffness_scale2(int n, const double* __restrict__ rho,
+                                                              double penal, double* __restrict__ s64,
+                                                              float* __restrict__ s32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double v = kEmin + (1.0 - kEmin) * pow(rho[e], penal);
+  s64[e] = v;
+  s32[e] = (float)v;
+}
+
 // s_c[E] = 1/4 * sum of the (in-domain) children scales.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_coarsen_scale(Grid f, Grid c, const T* __restrict__ sf,
@@ -423,6 +434,12 @@ template <typename T>
 void launch_stiffness_scale(Plan& P, const double* rho, double penal, T* s_out, hipStream_t s) {
   hipLaunchKernelGGL(k_stiffness_scale<T>, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
                      rho, penal, s_out);
+}
+
+void launch_stiffness_scale2(Plan& P, const double* rho, double penal, double* s64, float* s32,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_stiffness_scale2, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g.nele,
+                     rho, penal, s64, s32);
 }
 
 template <typename T>

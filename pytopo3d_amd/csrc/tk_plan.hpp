@@ -144,6 +144,8 @@ void launch_count_free(Plan& P, hipStream_t s);
 // operator / multigrid
 template <typename T>
 void launch_stiffness_scale(Plan& P, const double* rho, double penal, T* s_out, hipStream_t s);
+void launch_stiffness_scale2(Plan& P, const double* rho, double penal, double* s64, float* s32,
+                             hipStream_t s);
 template <typename T>
 void launch_coarsen_scale(const Level& f, const Level& c, const T* sf, T* sc, hipStream_t s);
 void launch_coarsen_fixed(const Level& f, const Level& c, hipStream_t s);

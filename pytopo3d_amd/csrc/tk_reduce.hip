@@ -9,8 +9,15 @@
 
 namespace tk {
 
-__global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* __restrict__ partials,
-                                                  Scalars* sc, double a0, int i0) {
+constexpr int kFinThreadsMax = 1024;
+constexpr int kFinLoads = 8;   // independent loads in flight per thread
+
+// blockDim.x: 256, or 1024 for more than 256 * kFinLoads partials (one batch of
+// loads per thread either way up to 8192 partials; a 1024-thread block costs
+// ~1 us more to run, which the solver's per-iteration reductions would pay)
+__global__ __launch_bounds__(kFinThreadsMax) void k_finalize(int op, int G,
+                                                          const double* __restrict__ partials,
+                                                          Scalars* sc, double a0, int i0) {
   if ((op == kFinAlpha || op == kFinUpdate || op == kFinRz || op == kFinUpdateFlex ||
        op == kFinRzPR) && sc->done)
     return;
@@ -18,14 +25,42 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
   double s[3] = {0.0, 0.0, 0.0};
   double m = -INFINITY;
   const bool two = (op == kFinNorms || op == kFinStats || op == kFinUpdateFlex);
-  if (op == kFinRatioMax) {
-    for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmax(m, partials[b]);
-  } else {
-#pragma unroll 4
-    for (int b = threadIdx.x; b < G; b += blockDim.x) {
-      s[0] += partials[b];
-      if (two) s[1] += partials[G + b];
-      if (op == kFinStats) m = fmax(m, partials[2 * G + b]);
+  // The partials were just written by other compute units, so every load is a
+  // full memory round trip: each thread keeps kFinLoads independent loads in
+  // flight (into separate accumulators, combined in a fixed order).
+  {
+    double a[kFinLoads], c[kFinLoads], x[kFinLoads];
+#pragma unroll
+    for (int j = 0; j < kFinLoads; ++j) {
+      a[j] = c[j] = 0.0;
+      x[j] = -INFINITY;
+    }
+    const bool is_max = op == kFinRatioMax, stats = op == kFinStats;
+    const int T = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < G; b0 += kFinLoads * T) {
+      // unconditional loads at clamped indices (G >= 1), masked afterwards
+      double p[kFinLoads], q[kFinLoads], r[kFinLoads];
+#pragma unroll
+      for (int j = 0; j < kFinLoads; ++j) {
+        const int b = min(b0 + j * T, G - 1);
+        p[j] = partials[b];
+        q[j] = two ? partials[G + b] : 0.0;
+        r[j] = stats ? partials[2 * G + b] : -INFINITY;
+      }
+#pragma unroll
+      for (int j = 0; j < kFinLoads; ++j) {
+        const bool in = b0 + j * T < G;
+        if (is_max) x[j] = fmax(x[j], in ? p[j] : -INFINITY);
+        else a[j] += in ? p[j] : 0.0;
+        c[j] += in ? q[j] : 0.0;
+        x[j] = fmax(x[j], in ? r[j] : -INFINITY);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kFinLoads; ++j) {
+      s[0] += a[j];
+      s[1] += c[j];
+      m = fmax(m, x[j]);
     }
   }
   block_sum<3>(s);
@@ -104,7 +139,8 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int G, const double* _
 }
 
 void finalize(Plan& P, int op, int G, hipStream_t s, double a0, int i0) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, op, G, P.partials, P.sc, a0, i0);
+  const int threads = G > 256 * kFinLoads ? kFinThreadsMax : 256;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(threads), 0, s, op, G, P.partials, P.sc, a0, i0);
 }
 
 }  // namespace tk

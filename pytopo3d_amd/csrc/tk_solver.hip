@@ -572,8 +572,10 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
 // ------------------------------------------------------------------ hierarchy
 int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStream_t s) {
   Level& L0 = P.lv[0];
-  launch_stiffness_scale<double>(P, rho, penal, L0.f64.s, s);
-  if (prec != 0) launch_stiffness_scale<float>(P, rho, penal, L0.f32.s, s);
+  // FP64 scales in either precision (the FP64 outer operator and the coarsest
+  // factorisation read them); the FP32 copy from the same pass
+  if (prec != 0) launch_stiffness_scale2(P, rho, penal, L0.f64.s, L0.f32.s, s);
+  else launch_stiffness_scale<double>(P, rho, penal, L0.f64.s, s);
   for (int l = 0; l + 1 < P.nlev; ++l) {
     launch_coarsen_scale<double>(P.lv[l], P.lv[l + 1], P.lv[l].f64.s, P.lv[l + 1].f64.s, s);
     if (prec != 0)
@@ -589,10 +591,11 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   launch_coarse_setup(P, P.cstream);
   TK_HIP(hipEventRecord(P.cev_out, P.cstream));
   P.coarse_pending = true;
+  // inverse diagonals for the V-cycle (and Jacobi) of this precision only
   for (int l = 0; l < P.nlev; ++l) {
     Level& L = P.lv[l];
-    launch_diag_inv<double>(P, L, L.f64.s, L.f64.dinv, (double*)nullptr, s);
-    if (prec != 0) launch_diag_inv<float>(P, L, L.f32.s, L.f32.dinv, (float*)nullptr, s);
+    if (prec == 0) launch_diag_inv<double>(P, L, L.f64.s, L.f64.dinv, (double*)nullptr, s);
+    else launch_diag_inv<float>(P, L, L.f32.s, L.f32.dinv, (float*)nullptr, s);
   }
   TK_HIP(hipGetLastError());
   P.last_penal = penal;

@@ -319,12 +319,13 @@ def test_prepare_then_reuse_equals_solve(precision):
     P.prepare(rho, 3.0, precision=precision)
     P.solve(rho, 3.0, F, Ub, rtol=1e-11, precision=precision, reuse=True)
     assert torch.equal(Ua, Ub)
-    P.prepare(rho, 3.0, precision="fp64")
-    if precision == "mixed":
-        with pytest.raises(ValueError):
-            P.solve(rho, 3.0, F, Ub, rtol=1e-11, precision="mixed", reuse=True)
+    # a set-up builds the state of its own precision only
+    other = "fp64" if precision == "mixed" else "mixed"
+    P.prepare(rho, 3.0, precision=other)
     with pytest.raises(ValueError):
-        P.solve(rho, 2.0, F, Ub, rtol=1e-11, precision="fp64", reuse=True)
+        P.solve(rho, 3.0, F, Ub, rtol=1e-11, precision=precision, reuse=True)
+    with pytest.raises(ValueError):
+        P.solve(rho, 2.0, F, Ub, rtol=1e-11, precision=other, reuse=True)
     P.close()
 
 
