@@ -46,7 +46,8 @@ enum {
 enum {
   TK_SOLVE_X0 = 1,        /* U holds the initial guess (warm start) */
   TK_SOLVE_JACOBI = 2,    /* Jacobi instead of the multigrid preconditioner */
-  TK_SOLVE_REUSE = 4      /* reuse the hierarchy built by the last solve (same rho, penal) */
+  TK_SOLVE_REUSE = 4      /* reuse the hierarchy built by the last solve or tk_prepare (same
+                             rho, penal and precision) */
 };
 
 typedef struct tk_options {
@@ -137,9 +138,10 @@ int tk_diag_K(tk_plan* plan, const double* rho, double penal, double* diag, void
 
 /* Build the solver state of a design without solving: element scales of every
  * multigrid level, inverse diagonals and the coarsest-grid inverse, in the
- * given precision (TK_MIXED_FP32 builds both).  A following tk_solve with
- * TK_SOLVE_REUSE (same rho and penal), tk_apply_K with rho == NULL and
- * tk_smoother_step use it.  This is the set-up half of the solve at
+ * given precision (TK_MIXED_FP32 also forms the FP64 element scales, which its
+ * FP64 outer iteration reads, but no FP64 inverse diagonals).  A following
+ * tk_solve with TK_SOLVE_REUSE (same rho, penal and precision), tk_apply_K with
+ * rho == NULL and tk_smoother_step (same precision) use it.  This is the set-up half of the solve at
  * pytopo3d/core/optimizer.py:584-594; it lets callers with several load cases
  * per design build it once. */
 int tk_prepare(tk_plan* plan, int prec, const double* rho, double penal, void* stream);
