@@ -305,16 +305,12 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     }
   };
 
-  // the operator-only sweeps run at higher occupancy, which hides these loads
-  // without the registers of a second set
-  constexpr bool kPrefetch = MODE != kModeApply && MODE != kModeApplyDir;
-
   double dot = 0.0;
   PlaneRegs<T, MODE> RA, RB;
   LayerIn<T> LA, LB;
   fetch(Z0 - 1, RB);
   fetch(Z0, RA);
-  if constexpr (kPrefetch) fetch_layer(Z0 - 1, LB);
+  fetch_layer(Z0 - 1, LB);
   // the convergence flag is read with the first loads in flight
   if (done && *done) return;
   store(0, RB);
@@ -339,16 +335,15 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   // pre receives plane ez + 2.  Consecutive layers swap the roles of the
   // buffer pairs, so the z-loop is unrolled by two and nothing is copied.
   auto layer = [&](int ez, int buf, T (&Fb)[3][4], T (&Ft)[3][4], const PlaneRegs<T, MODE>& nxt,
-                   PlaneRegs<T, MODE>& pre, const LayerIn<T>& cur, LayerIn<T>& nx) {
-    // issue the global loads of the next layer first
+                   PlaneRegs<T, MODE>& pre, const LayerIn<T>& lin, LayerIn<T>& nx) {
+    // the next plane and the next layer's scale and epilogue operands; this
+    // layer's were loaded one layer ago (a load issued and used in the same
+    // layer would, through the in-order load counter, wait for these too)
     const bool more = ez + 1 < Z1;
     if (more) {
       fetch(ez + 2, pre);
-      if constexpr (kPrefetch) fetch_layer(ez + 1, nx);
+      fetch_layer(ez + 1, nx);
     }
-    LayerIn<T> here;
-    if constexpr (!kPrefetch) fetch_layer(ez, here);
-    const LayerIn<T>& lin = kPrefetch ? cur : here;
     const bool out_now = owner && ez >= Z0;
     const int n_out = ey + ex * g.nny + ez * nxy;
     T eb[3], ed[3], edi[3];
@@ -360,7 +355,13 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     }
     const T se = lin.se;
 
-    // forward transform, sector blocks, inverse z butterfly
+    // forward transform, sector blocks, inverse z butterfly.  FP64 operator
+    // sweeps (3 waves per SIMD): the bottom face spectrum is formed again from
+    // its plane, still in LDS, instead of being carried from the previous
+    // layer (12 doubles fewer live across the sector blocks, which spilled);
+    // the FP64 smoothing sweeps run at 2 waves without spilling and carry it
+    if constexpr (sizeof(T) == 8 && (MODE == kModeApply || MODE == kModeApplyDir))
+      face_spectrum(buf ^ 1, Fb);
     face_spectrum(buf, Ft);
     T a[3][8], o[3][8];
 #pragma unroll
