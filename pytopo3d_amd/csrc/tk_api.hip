@@ -87,7 +87,8 @@ static void free_plan(Plan* P) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int c = 0; c < Plan::kChunkSizes; ++c)
-        if (P->gexec[a][b][c]) (void)hipGraphExecDestroy(P->gexec[a][b][c]);
+        for (int d = 0; d < 2; ++d)
+          if (P->gexec[a][b][c][d]) (void)hipGraphExecDestroy(P->gexec[a][b][c][d]);
   if (P->gev_in) (void)hipEventDestroy(P->gev_in);
   if (P->gev_out) (void)hipEventDestroy(P->gev_out);
   if (P->gstream) (void)hipStreamDestroy(P->gstream);

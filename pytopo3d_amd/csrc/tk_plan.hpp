@@ -111,9 +111,10 @@ struct Plan {
   hipStream_t cstream = nullptr;
   hipEvent_t cev_in = nullptr, cev_out = nullptr;
   bool coarse_pending = false;
-  static constexpr int kChunkSizes = 4;   // chunk lengths 2 * base, base, base / 2, 2
-  hipGraphExec_t gexec[2][2][kChunkSizes] = {};
-  Counters gdelta[2][2][kChunkSizes];      // host counters added by one replay
+  static constexpr int kChunkSizes = 5;   // chunk lengths 2 * base, base, base / 2, 2, 1
+  // [precision][jacobi][length][parity of the chunk's first iteration]
+  hipGraphExec_t gexec[2][2][kChunkSizes][2] = {};
+  Counters gdelta[2][2][kChunkSizes][2];   // host counters added by one replay
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
 };
 

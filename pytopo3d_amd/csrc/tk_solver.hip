@@ -354,19 +354,26 @@ static void add_counters(Counters& a, const Counters& d, int sign) {
   a.filter_passes += sign * d.filter_passes;
 }
 
-// Runs `body(stream, m)` -- m Krylov iterations whose launches all return at
-// once after the device-side `done` flag is set -- in chunks until the flag is
-// set, reading the device scalars back after every chunk.  Chunk lengths are
-// taken from {2 base, base, base / 2, 2} (base = check_every rounded up to an
-// even count): the next length is the largest one not above the iterations
-// still needed, predicted from the residual decay of the previous chunk, so
-// few iterations run past convergence.  With graphs enabled every chunk
-// length is captured on the plan's stream the first time it is used and
-// replayed from then on (the kernels, their order and arguments are the same
-// in every chunk of that length), so the device runs it without host launch
-// gaps.
+// Runs `body(stream, m, par)` -- m Krylov iterations, the first of parity par,
+// whose launches all return at once after the device-side `done` flag is set --
+// in chunks until the flag is set, reading the device scalars back after every
+// chunk.  Chunk lengths are taken from {2 base, base, base / 2, 2, 1} (base =
+// check_every rounded up to an even count): the next length is the largest one
+// not above the iterations still needed, predicted from the residual decay
+// since the previous read (from the initial residual for the first), so few
+// iterations run past convergence (each costs some 65 launches that return at
+// once, ~0.5 ms at 256x128x128, against ~50 us for one more read): the
+// prediction aims at three quarters of the estimate, since the convergence of
+// CG speeds up.  The parity matters to bodies that alternate buffers between
+// iterations (par0 >= 0); it starts at par0 and flips after every chunk of odd
+// length (par0 < 0: parity-free body, one graph per length).  With graphs
+// enabled every (length, parity) is captured on the plan's stream before the
+// first chunk runs and replayed from then on (the kernels, their order and
+// arguments are the same in every such chunk), so the device runs it without
+// host launch gaps.
 template <class Body>
-static int run_chunks(Plan& P, int prec, int jacobi, int maxit, Body body, hipStream_t s) {
+static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body body,
+                      hipStream_t s) {
   bool graphs = !P.opt.no_graphs;
   if (graphs && !P.gstream) {
     if (hipStreamCreateWithFlags(&P.gstream, hipStreamNonBlocking) != hipSuccess ||
@@ -378,17 +385,17 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, Body body, hipSt
   }
   int base = P.opt.check_every > 0 ? P.opt.check_every : 8;
   base += base & 1;
-  int sizes[Plan::kChunkSizes] = {2 * base, base, base / 2 + ((base / 2) & 1), 2};
+  int sizes[Plan::kChunkSizes] = {2 * base, base, base / 2 + ((base / 2) & 1), 2, 1};
   const int pa = prec ? 1 : 0, ja = jacobi ? 1 : 0;
-  // capture every chunk length at the first use, so later solves never pay it
-  auto capture = [&](int k) -> int {
+  // capture a (length, parity) at its first use, so later solves never pay it
+  auto capture = [&](int k, int par) -> int {
     const int m = sizes[k];
-    hipGraphExec_t& ex = P.gexec[pa][ja][k];
-    Counters& delta = P.gdelta[pa][ja][k];
+    hipGraphExec_t& ex = P.gexec[pa][ja][k][par];
+    Counters& delta = P.gdelta[pa][ja][k][par];
     if (!ex) {
       const Counters before = P.cnt;
       TK_HIP(hipStreamBeginCapture(P.gstream, hipStreamCaptureModeThreadLocal));
-      body(P.gstream, m);
+      body(P.gstream, m, par);
       hipGraph_t g = nullptr;
       TK_HIP(hipStreamEndCapture(P.gstream, &g));
       const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -404,16 +411,18 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, Body body, hipSt
     }
     return 0;
   };
+  const bool parity = par0 >= 0;
   if (graphs)
     for (int k = 0; k < Plan::kChunkSizes; ++k)
-      if (int rc = capture(k)) return rc;
-  auto run = [&](int k) -> int {
+      for (int q = 0; q < (parity ? 2 : 1); ++q)
+        if (int rc = capture(k, q)) return rc;
+  auto run = [&](int k, int par) -> int {
     if (!graphs) {
-      body(s, sizes[k]);
+      body(s, sizes[k], par);
       return 0;
     }
-    hipGraphExec_t& ex = P.gexec[pa][ja][k];
-    Counters& delta = P.gdelta[pa][ja][k];
+    hipGraphExec_t& ex = P.gexec[pa][ja][k][par];
+    Counters& delta = P.gdelta[pa][ja][k][par];
     TK_HIP(hipEventRecord(P.gev_in, s));
     TK_HIP(hipStreamWaitEvent(P.gstream, P.gev_in, 0));
     TK_HIP(hipGraphLaunch(ex, P.gstream));
@@ -423,19 +432,24 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, Body body, hipSt
     return 0;
   };
   int k = 1;                       // first chunk: base iterations
+  int par = parity ? par0 : 0;
+  // the decay is measured from the initial residual for the first prediction
+  // (kFinNorms left it in rr0; the iteration count starts at 0)
   int it_prev = 0;
-  double rr_prev = 0.0;
+  double rr_prev = -1.0;
   for (long long guard = 0; guard < (long long)maxit + 8; ++guard) {
-    if (int rc = run(k)) return rc;
+    if (int rc = run(k, par)) return rc;
+    if (parity) par ^= sizes[k] & 1;
     if (read_scalars(P, s)) return 2;
     const Scalars& H = *P.sc_host;
     if (H.done) break;
+    if (rr_prev < 0.0) rr_prev = H.rr0;
     // iterations still needed at the observed residual decay rate
     const double target = H.tol2 * H.bb;
     int need = sizes[0];
     if (it_prev < H.iter && rr_prev > 0.0 && H.rr > 0.0 && H.rr < rr_prev && target > 0.0) {
       const double per_it = std::log(H.rr / rr_prev) / (double)(H.iter - it_prev);
-      const double est = std::log(target / H.rr) / per_it;
+      const double est = 0.75 * std::log(target / H.rr) / per_it;
       if (est < (double)sizes[0]) need = est < 1.0 ? 1 : (int)std::ceil(est);
     }
     it_prev = H.iter;
@@ -478,7 +492,7 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   const T* z = precondition<T>(P, r, q, jacobi, s, &G);
   reduce_rz(z, G, 1, s);
   hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, p, 1, done, P.sc);
-  auto body = [&](hipStream_t st, int m) {
+  auto body = [&](hipStream_t st, int m, int) {
     for (int k = 0; k < m; ++k) {
       launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, st);
       hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, p, q,
@@ -492,7 +506,7 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
                          P.sc);
     }
   };
-  int rc = run_chunks(P, 0, jacobi, maxit, body, s);
+  int rc = run_chunks(P, 0, jacobi, maxit, -1, body, s);
   if (rc) return rc;
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in pcg");
@@ -541,8 +555,8 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
   };
   const float* z32 = correction(1, s);
   // iteration i writes direction pbuf[i & 1] and reads pbuf[(i - 1) & 1]
-  // (none for i = 0); chunks of even length start at odd i, so every chunk
-  // launches the same kernels with the same arguments.
+  // (none for i = 0); a chunk's kernels and arguments depend on its length
+  // and the parity of its first iteration (run_chunks tracks it).
   auto iteration = [&](float* pnew, const float* pold, hipStream_t st) {
     if (P.use_el) {
       el_apply_dir(P, z32, pold, pnew, q, st);
@@ -557,10 +571,14 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
     z32 = correction(0, st);
   };
   iteration(pbuf[0], nullptr, s);
-  auto body = [&](hipStream_t st, int m) {   // m is even
-    for (int k = 0; k < m; ++k) iteration(pbuf[(k + 1) & 1], pbuf[k & 1], st);
+  auto body = [&](hipStream_t st, int m, int par) {
+    for (int k = 0; k < m; ++k) {
+      const int i = par + k;   // parity of the global iteration index
+      iteration(pbuf[i & 1], pbuf[(i + 1) & 1], st);
+    }
   };
-  int rc = run_chunks(P, 1, jacobi, maxit, body, s);
+  // iteration 0 ran eagerly: the chunks start at an odd index
+  int rc = run_chunks(P, 1, jacobi, maxit, 1, body, s);
   if (rc) return rc;
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in mixed pcg");
