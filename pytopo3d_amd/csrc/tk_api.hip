@@ -283,6 +283,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     return 2;
   }
   if (P->opt.filter_kind == 1 || !filter_tiled_setup(*P, off, w)) P->ftiled = false;
+  tail_setup(*P);
   launch_filter_hs(*P, s);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();

@@ -116,6 +116,18 @@ struct Plan {
   hipGraphExec_t gexec[2][2][kChunkSizes][2] = {};
   Counters gdelta[2][2][kChunkSizes][2];   // host counters added by one replay
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
+  int tail_level = -1;            // level whose V-cycle part runs in one workgroup (-1: none)
+};
+
+// Chebyshev coefficients of one smoothing pass on the single-workgroup tail
+// (step 0: d = c0 D^-1 r; step k >= 1: d = c1[k] d + c2[k] D^-1 r)
+constexpr int kTailMaxDeg = 8;
+constexpr int kTailMaxNodes = 512;
+struct TailCoef {
+  int deg;
+  double c0;
+  double c1[kTailMaxDeg];
+  double c2[kTailMaxDeg];
 };
 
 // ---- two-phase reductions (tk_reduce.hip) -----------------------------------
@@ -206,6 +218,11 @@ void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const
                      const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,
                      const int* done, hipStream_t s);
 void launch_coarse_setup(Plan& P, hipStream_t s);
+// the second-coarsest level's V-cycle part and the coarsest solve in one
+// workgroup (tk_tail.hip); tail_setup picks the level when it fits
+void tail_setup(Plan& P);
+template <typename T>
+void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);
 // energy

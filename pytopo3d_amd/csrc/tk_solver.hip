@@ -273,6 +273,26 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
   return x;
 }
 
+// The coefficients smooth() uses on level l (same recurrence), for the tail.
+static TailCoef tail_coefs(const Plan& P, int l) {
+  const bool coarse = l >= 2;
+  TailCoef cf{};
+  cf.deg = coarse ? P.opt.coarse_degree : P.opt.smooth_degree;
+  const double lmax = P.lam_bound * P.opt.cheb_hi;
+  const double lmin = P.lam_bound * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  cf.c0 = 1.0 / theta;
+  for (int k = 1; k < cf.deg && k < kTailMaxDeg; ++k) {
+    const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+    cf.c1[k] = rho_new * rho_old;
+    cf.c2[k] = 2.0 * rho_new / delta;
+    rho_old = rho_new;
+  }
+  return cf;
+}
+
 // Make s wait for the coarsest factorisation of the last set-up, once.  The
 // first coarsest solve after a set-up is always launched eagerly (before any
 // Krylov chunk is captured), so no captured graph holds this wait.
@@ -292,7 +312,10 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
   T* alt[kMaxLevels];
   const T* rhs[kMaxLevels];
   rhs[0] = b0;
-  for (int l = 0; l < nl - 1; ++l) {
+  // from the tail level down, one workgroup runs the rest of the cycle
+  const int lt = P.tail_level;
+  const int descend_to = lt >= 1 ? lt : nl - 1;
+  for (int l = 0; l < descend_to; ++l) {
     LevelBuf<T>& B = buf<T>(P.lv[l]);
     T* x = smooth<T>(P, l, rhs[l], B.x, B.xa, 1, s);
     cur[l] = x;
@@ -302,13 +325,18 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
     launch_restrict<T>(P.lv[l], P.lv[l + 1], B.r, C.b, s, done);
     rhs[l + 1] = C.b;
   }
-  {
+  if (lt >= 1) {
+    LevelBuf<T>& C = buf<T>(P.lv[lt]);
+    join_coarse(P, s);   // the tail reads the coarsest inverse
+    launch_tail<T>(P, rhs[lt], C.x, tail_coefs(P, lt), s);
+    cur[lt] = C.x;
+  } else {
     LevelBuf<T>& C = buf<T>(P.lv[nl - 1]);
     join_coarse(P, s);
     launch_coarse_solve<T>(P, rhs[nl - 1], C.x, s);
     cur[nl - 1] = C.x;
   }
-  for (int l = nl - 2; l >= 0; --l) {
+  for (int l = descend_to - 1; l >= 0; --l) {
     // small levels: the prolongation rides in the first smoothing sweep (one
     // launch less); large levels: the separate sweep is faster than gathering
     // the coarse correction inside the register-heavy smoothing kernel
