@@ -5,7 +5,12 @@
 //   chain-rule weights (g * drho)/Hs for g = dc/drho and g = free mask
 //                                                 pytopo3d/utils/filter.py:115-118
 // The transposed filter that finishes the chain rule is k_filter_transpose2.
+// When KE has the mirror symmetry of the element kernel the energy is formed in
+// the Walsh-Hadamard sector form, u.KE.u = a.B.a with a the corner transform
+// of u and B the 45 sector entries (21 modes, ~170 operations per element
+// instead of the 1152 of the dense product).
 #include "tk_plan.hpp"
+#include "tk_sector.hpp"
 
 namespace tk {
 
@@ -56,12 +61,74 @@ __global__ __launch_bounds__(kBlock) void k_energy(
   block_partials<1>(part, partials);
 }
 
+__global__ __launch_bounds__(kBlock) void k_energy_sector(
+    Grid g, SparseKE<double> kb, const double* __restrict__ U, const double* __restrict__ rho,
+    const double* __restrict__ drho, const double* __restrict__ Hs,
+    const uint8_t* __restrict__ fl, double penal, double c0, double* __restrict__ ce_out,
+    double* __restrict__ dc_out, double* __restrict__ wc, double* __restrict__ wv,
+    double* partials, Scalars* sc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double part[1] = {0.0};
+  if (e < g.nele) {
+    const int iy = e % g.nely, t = e / g.nely, ix = t % g.nelx, iz = t / g.nelx;
+    const int nxy = g.nnx * g.nny;
+    double Fb[3][4], Ft[3][4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int m0 = (iy + (f >> 1)) + (ix + (f & 1)) * g.nny + iz * nxy;
+      const int m1 = m0 + nxy;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Fb[c][f] = U[3 * m0 + c];
+        Ft[c][f] = U[3 * m1 + c];
+      }
+    }
+    double a[3][8], o[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      face_fwd(Fb[c]);
+      face_fwd(Ft[c]);
+      a[c][0] = 0.0;
+#pragma unroll
+      for (int f = 1; f < 4; ++f) a[c][f] = Fb[c][f] + Ft[c][f];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
+    }
+    sector_apply<double>(kb.v, 1.0, a, o);
+    double ce = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 1; m < 8; ++m) ce += a[c][m] * o[c][m];
+    const double r = rho[e];
+    const double s = kEminE + (1.0 - kEminE) * pow(r, penal);
+    part[0] = s * ce;
+    const double dc = c0 * pow(r, penal - 1.0) * ce;
+    if (ce_out) ce_out[e] = ce;
+    if (dc_out) dc_out[e] = dc;
+    if (wc) {
+      const double d = drho[e], h = Hs[e];
+      wc[e] = (dc * d) / h;
+      wv[e] = ((fl[e] == 0 ? 1.0 : 0.0) * d) / h;
+    }
+  }
+  block_partials<1>(part, partials);
+}
+
 void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
                    double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s) {
   const double c0 = -penal * (1.0 - kEminE);
-  hipLaunchKernelGGL(k_energy, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.KE64, U, rho,
-                     drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials,
-                     P.sc);
+  if (P.sector_ok) {
+    SparseKE<double> kb;
+    for (int k = 0; k < kSecN; ++k) kb.v[k] = P.sec64[k];
+    hipLaunchKernelGGL(k_energy_sector, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, kb, U,
+                       rho, drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials,
+                       P.sc);
+  } else {
+    hipLaunchKernelGGL(k_energy, dim3(nblocks(P.g.nele)), dim3(kBlock), 0, s, P.g, P.KE64, U,
+                       rho, drho, P.Hs, P.eflags, penal, c0, ce, dc_drho, wc, wv, P.partials,
+                       P.sc);
+  }
   finalize(P, kFinC, nblocks(P.g.nele), s);
 }
 

@@ -200,13 +200,21 @@ def test_solve_true_residual_across_tiles(precision):
     P.close()
 
 
+@pytest.mark.parametrize("ke", ["h8", "asymmetric"])
 @pytest.mark.parametrize("nelx,nely,nelz,rmin", [(6, 4, 3, 1.5), (10, 6, 4, 2.5)])
-def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin):
-    note("energy", nelx, nely, nelz, rmin)
+def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin, ke):
+    """The H8 KE takes the Walsh-Hadamard sector form of the energy; a KE
+    without its mirror symmetry (a symmetric random perturbation) takes the
+    dense 24x24 product."""
+    note("energy", nelx, nely, nelz, rmin, ke)
     rng = np.random.default_rng(5)
     shape = (nely, nelx, nelz)
     solid, void = random_masks(rng, shape, 0.05, 0.05)
-    P, KE = make_plan(nelx, nely, nelz, rmin, solid, void)
+    KE = oracle.element_stiffness(None, 1.0)
+    if ke == "asymmetric":
+        R = rng.standard_normal((24, 24))
+        KE = KE + 1e-3 * np.abs(KE).max() * (R + R.T)
+    P, KE = make_plan(nelx, nely, nelz, rmin, solid, void, KE=KE)
     H, Hs = oracle.filter_matrix(nelx, nely, nelz, rmin)
     x = rng.uniform(0.05, 0.95, shape)
     x[solid], x[void] = 1.0, 0.0
