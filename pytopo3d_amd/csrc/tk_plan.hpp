@@ -117,6 +117,8 @@ struct Plan {
   Counters gdelta[2][2][kChunkSizes][2];   // host counters added by one replay
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
   int tail_level = -1;            // level whose V-cycle part runs in one workgroup (-1: none)
+  bool tail_inv_lds64 = false;    // the tail keeps a copy of the coarsest inverse in LDS
+  bool tail_inv_lds32 = false;
 };
 
 // Chebyshev coefficients of one smoothing pass on the single-workgroup tail

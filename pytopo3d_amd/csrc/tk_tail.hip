@@ -24,13 +24,21 @@ namespace tk {
 
 constexpr int kTailThreads = 256;
 
-// LDS bytes of the tail for level g and coarsest grid gc (element type T):
+// LDS layout of the tail for level g and coarsest grid gc (element type T):
 // sector entries, x (two buffers), d, b, D^-1, element scales, the element
-// outputs (24 per element), coarse right-hand side and solution, then the
-// fixed bits of both grids.
+// outputs (24 per element), coarse right-hand side and solution, optionally a
+// copy of the coarsest inverse (16-byte aligned), then the fixed bits of both
+// grids.  Offsets in elements of T.
 template <typename T>
-static size_t tail_lds(const Grid& g, const Grid& gc) {
+__host__ __device__ inline size_t tail_inv_offset(const Grid& g, const Grid& gc) {
   const size_t vals = 48 + 5 * (size_t)g.ndof + g.nele + 24 * (size_t)g.nele + 2 * (size_t)gc.ndof;
+  constexpr size_t a = 16 / sizeof(T);
+  return (vals + a - 1) / a * a;
+}
+
+template <typename T>
+static size_t tail_lds(const Grid& g, const Grid& gc, bool inv_lds) {
+  const size_t vals = tail_inv_offset<T>(g, gc) + (inv_lds ? (size_t)gc.ndof * gc.ndof : 0);
   return vals * sizeof(T) + (size_t)g.nnode + (size_t)gc.nnode;
 }
 
@@ -39,25 +47,38 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     Grid g, Grid gc, const T* __restrict__ sec, const T* __restrict__ sl,
     const uint8_t* __restrict__ fixed, const uint8_t* __restrict__ fixed_c,
     const T* __restrict__ dinv, const T* __restrict__ b, const T* __restrict__ inv,
-    T* __restrict__ xout, TailCoef cf, const int* __restrict__ done) {
+    T* __restrict__ xout, TailCoef cf, int inv_lds, const int* __restrict__ done) {
   if (*done) return;
   extern __shared__ __align__(16) unsigned char tail_mem[];
   T* kb = reinterpret_cast<T*>(tail_mem);   // 45 sector entries (48 slots)
-  T* X[2];
-  X[0] = kb + 48;
-  X[1] = X[0] + g.ndof;
-  T* D = X[1] + g.ndof;
+  // x, two buffers; chosen by offset (a runtime-indexed pointer array loses
+  // the LDS address space and turns every access into a flat one)
+  T* X0 = kb + 48;
+  auto X = [&](int w) -> T* { return X0 + (w ? g.ndof : 0); };
+  T* D = X0 + 2 * g.ndof;
   T* B = D + g.ndof;
   T* DI = B + g.ndof;
   T* S = DI + g.ndof;
   T* EY = S + g.nele;          // element outputs: [element][corner f + 4 oz][component]
   T* BC = EY + 24 * g.nele;
   T* XC = BC + gc.ndof;
-  uint8_t* F = reinterpret_cast<uint8_t*>(XC + gc.ndof);
+  T* IV = kb + tail_inv_offset<T>(g, gc);   // the coarsest inverse, when inv_lds
+  uint8_t* F = reinterpret_cast<uint8_t*>(IV + (inv_lds ? gc.ndof * gc.ndof : 0));
   uint8_t* FC = F + g.nnode;
   const int tid = threadIdx.x, nt = blockDim.x;
   const T c0 = (T)cf.c0;
+  const int n2 = gc.ndof * gc.ndof;
+  if (inv_lds) {
+    // 16-byte copies, several in flight per thread
+    const int n16 = n2 * (int)sizeof(T) / 16;
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(inv);
+    uint4* dst = reinterpret_cast<uint4*>(IV);
+#pragma unroll 8
+    for (int i = tid; i < n16; i += nt) dst[i] = src[i];
+    for (int i = n16 * (16 / (int)sizeof(T)) + tid; i < n2; i += nt) IV[i] = inv[i];
+  }
   for (int i = tid; i < kSecN; i += nt) kb[i] = sec[i];
+#pragma unroll 4
   for (int i = tid; i < g.ndof; i += nt) {
     B[i] = b[i];
     DI[i] = dinv[i];
@@ -142,11 +163,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   };
 
   int cur = 0;
-  // one Chebyshev step from X[cur] into X[cur ^ 1]: d = c1 d + c2 D^-1 (b - K x)
+  // one Chebyshev step from X(cur) into X(cur ^ 1): d = c1 d + c2 D^-1 (b - K x)
   // (first: d = c2 D^-1 (b - K x))
   auto step = [&](bool first, T c1, T c2) {
-    const T* x = X[cur];
-    T* xo = X[cur ^ 1];
+    const T* x = X(cur);
+    T* xo = X(cur ^ 1);
     element_products(x);
     for (int n = tid; n < g.nnode; n += nt) {
       T acc[3];
@@ -169,15 +190,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   for (int i = tid; i < g.ndof; i += nt) {
     const T dd = c0 * DI[i] * B[i];
     D[i] = dd;
-    X[cur][i] = dd;
+    X(cur)[i] = dd;
   }
   __syncthreads();
   for (int k = 1; k < cf.deg; ++k) step(false, (T)cf.c1[k], (T)cf.c2[k]);
 
   // residual (0 on fixed DOFs) into the free buffer, restricted to the coarsest grid
   {
-    const T* x = X[cur];
-    T* r = X[cur ^ 1];
+    const T* x = X(cur);
+    T* r = X(cur ^ 1);
     element_products(x);
     for (int n = tid; n < g.nnode; n += nt) {
       T acc[3];
@@ -201,22 +222,32 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   }
 
   // coarsest solve with the stored inverse (symmetric: row i read as column i,
-  // coalesced over the threads), accumulated in double
-  for (int i = tid; i < gc.ndof; i += nt) {
-    double acc = 0.0;
+  // coalesced over the threads), accumulated in double; from the LDS copy when
+  // it fits, else from global memory with many loads in flight
+  if (inv_lds) {
+    for (int i = tid; i < gc.ndof; i += nt) {
+      double acc = 0.0;
 #pragma unroll 8
-    for (int j = 0; j < gc.ndof; ++j) acc += (double)inv[j * gc.ndof + i] * (double)BC[j];
-    XC[i] = (T)acc;
+      for (int j = 0; j < gc.ndof; ++j) acc += (double)IV[j * gc.ndof + i] * (double)BC[j];
+      XC[i] = (T)acc;
+    }
+  } else {
+    for (int i = tid; i < gc.ndof; i += nt) {
+      double acc = 0.0;
+#pragma unroll 16
+      for (int j = 0; j < gc.ndof; ++j) acc += (double)inv[j * gc.ndof + i] * (double)BC[j];
+      XC[i] = (T)acc;
+    }
   }
   __syncthreads();
 
   // prolongation onto the free DOFs, then the post-smoothing
-  for (int n = tid; n < g.nnode; n += nt) prolong_node(gc, g, F, XC, X[cur], n);
+  for (int n = tid; n < g.nnode; n += nt) prolong_node(gc, g, F, XC, X(cur), n);
   __syncthreads();
   step(true, T(0), c0);
   for (int k = 1; k < cf.deg; ++k) step(false, (T)cf.c1[k], (T)cf.c2[k]);
 
-  for (int i = tid; i < g.ndof; i += nt) xout[i] = X[cur][i];
+  for (int i = tid; i < g.ndof; i += nt) xout[i] = X(cur)[i];
 }
 
 void tail_setup(Plan& P) {
@@ -228,14 +259,17 @@ void tail_setup(Plan& P) {
   const Grid& gc = P.lv[lt + 1].g;
   const int deg = lt >= 2 ? P.opt.coarse_degree : P.opt.smooth_degree;
   if (g.nnode > kTailMaxNodes || deg > kTailMaxDeg || gc.ndof > kCoarseMaxDof) return;
-  const size_t lds = tail_lds<double>(g, gc);
-  if ((int)lds > P.lds_limit) return;
-  if (hipFuncSetAttribute((const void*)k_tail<double>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
-      hipFuncSetAttribute((const void*)k_tail<float>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)tail_lds<float>(g, gc)) != hipSuccess)
+  const size_t lds64 = tail_lds<double>(g, gc, false), lds32 = tail_lds<float>(g, gc, false);
+  if ((int)lds64 > P.lds_limit) return;
+  // the coarsest inverse joins the LDS state where it fits
+  P.tail_inv_lds64 = (int)tail_lds<double>(g, gc, true) <= P.lds_limit;
+  P.tail_inv_lds32 = (int)tail_lds<float>(g, gc, true) <= P.lds_limit;
+  if (hipFuncSetAttribute((const void*)k_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)tail_lds<double>(g, gc, P.tail_inv_lds64)) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)tail_lds<float>(g, gc, P.tail_inv_lds32)) != hipSuccess)
     return;   // the level-by-level cycle stays in use
+  (void)lds32;
   P.tail_level = lt;
 }
 
@@ -247,17 +281,21 @@ void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s) {
   const LevelBuf<T>* B;
   const T* sec;
   const T* inv;
+  bool inv_lds;
   if constexpr (std::is_same_v<T, double>) {
     B = &L.f64;
     sec = P.sec64_dev;
     inv = P.coarse_inv64;
+    inv_lds = P.tail_inv_lds64;
   } else {
     B = &L.f32;
     sec = P.sec32_dev;
     inv = P.coarse_inv32;
+    inv_lds = P.tail_inv_lds32;
   }
-  hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), tail_lds<T>(L.g, C.g), s, L.g, C.g,
-                     sec, B->s, L.fixed, C.fixed, B->dinv, b, inv, x, cf, &P.sc->done);
+  hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), tail_lds<T>(L.g, C.g, inv_lds), s,
+                     L.g, C.g, sec, B->s, L.fixed, C.fixed, B->dinv, b, inv, x, cf, (int)inv_lds,
+                     &P.sc->done);
 }
 
 template void launch_tail<float>(Plan&, const float*, float*, const TailCoef&, hipStream_t);
