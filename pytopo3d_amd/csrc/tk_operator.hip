@@ -17,6 +17,8 @@
 #include "tk_plan.hpp"
 #include "tk_transfer.hpp"
 
+#include <type_traits>
+
 namespace tk {
 
 constexpr double kEmin = 1.0e-9;
@@ -226,6 +228,47 @@ __global__ __launch_bounds__(kBlock) void k_diag_inv(Grid g, const double* __res
   }
 }
 
+// k_diag_inv of every level in one launch: the blocks of level l are
+// [block0[l], block0[l + 1]).
+template <typename T>
+struct DiagLevels {
+  int nlev;
+  int block0[kMaxLevels + 1];
+  Grid g[kMaxLevels];
+  const T* s[kMaxLevels];
+  const uint8_t* fixed[kMaxLevels];
+  T* dinv[kMaxLevels];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag_inv_levels(const double* __restrict__ KE,
+                                                             DiagLevels<T> d) {
+  int l = 0;
+  while (l + 1 < d.nlev && (int)blockIdx.x >= d.block0[l + 1]) ++l;
+  const Grid g = d.g[l];
+  const int n = ((int)blockIdx.x - d.block0[l]) * blockDim.x + threadIdx.x;
+  if (n >= g.nnode) return;
+  const T* __restrict__ s = d.s[l];
+  int iy, ix, iz;
+  node_coords(g, n, iy, ix, iz);
+  const int sxy = g.nelx * g.nely;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int ey = iy - corner_y(a), ex = ix - corner_x(a), ez = iz - corner_z(a);
+    if ((unsigned)ey >= (unsigned)g.nely || (unsigned)ex >= (unsigned)g.nelx ||
+        (unsigned)ez >= (unsigned)g.nelz)
+      continue;
+    const double se = (double)s[ey + ex * g.nely + ez * sxy];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += se * KE[(3 * a + c) * 25];
+  }
+  const uint8_t f = d.fixed[l][n];
+  T* __restrict__ dinv = d.dinv[l];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dinv[3 * n + c] = (T)(1.0 / (((f >> c) & 1) ? 1.0 : acc[c]));
+}
+
 // s = Emin + (1 - Emin) rho^p  (pytopo3d/core/optimizer.py:584-586)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stiffness_scale(int n, const double* __restrict__ rho,
@@ -261,6 +304,30 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_scale(Grid f, Grid c, const 
       acc += (double)sf[ey + ex * f.nely + ez * f.nelx * f.nely];
   }
   sc[E] = (T)(0.25 * acc);
+}
+
+// k_coarsen_scale for the FP64 and the FP32 chain in one pass (each from its
+// own finer level, so both match the separate passes bit for bit).
+__global__ __launch_bounds__(kBlock) void k_coarsen_scale2(Grid f, Grid c,
+                                                            const double* __restrict__ sf64,
+                                                            const float* __restrict__ sf32,
+                                                            double* __restrict__ sc64,
+                                                            float* __restrict__ sc32) {
+  const int E = blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= c.nele) return;
+  const int Ey = E % c.nely, t = E / c.nely, Ex = t % c.nelx, Ez = t / c.nelx;
+  double a64 = 0.0, a32 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ey = 2 * Ey + (k & 1), ex = 2 * Ex + ((k >> 1) & 1), ez = 2 * Ez + (k >> 2);
+    if (ey < f.nely && ex < f.nelx && ez < f.nelz) {
+      const int e = ey + ex * f.nely + ez * f.nelx * f.nely;
+      a64 += sf64[e];
+      a32 += (double)sf32[e];
+    }
+  }
+  sc64[E] = 0.25 * a64;
+  sc32[E] = (float)(0.25 * a32);
 }
 
 __global__ __launch_bounds__(kBlock) void k_coarsen_fixed(Grid f, Grid c,
@@ -447,6 +514,38 @@ void launch_coarsen_scale(const Level& f, const Level& c, const T* sf, T* sc, hi
   hipLaunchKernelGGL(k_coarsen_scale<T>, dim3(nblocks(c.g.nele)), dim3(kBlock), 0, s, f.g, c.g, sf,
                      sc);
 }
+
+void launch_coarsen_scale2(Plan& P, hipStream_t s) {
+  for (int l = 0; l + 1 < P.nlev; ++l) {
+    const Level& f = P.lv[l];
+    const Level& c = P.lv[l + 1];
+    hipLaunchKernelGGL(k_coarsen_scale2, dim3(nblocks(c.g.nele)), dim3(kBlock), 0, s, f.g, c.g,
+                       f.f64.s, f.f32.s, c.f64.s, c.f32.s);
+  }
+}
+
+template <typename T>
+void launch_diag_inv_levels(Plan& P, hipStream_t s) {
+  DiagLevels<T> d{};
+  d.nlev = P.nlev;
+  int nb = 0;
+  for (int l = 0; l < P.nlev; ++l) {
+    const Level& L = P.lv[l];
+    const LevelBuf<T>* B;
+    if constexpr (std::is_same_v<T, double>) B = &L.f64;
+    else B = &L.f32;
+    d.block0[l] = nb;
+    d.g[l] = L.g;
+    d.s[l] = B->s;
+    d.fixed[l] = L.fixed;
+    d.dinv[l] = B->dinv;
+    nb += nblocks(L.g.nnode);
+  }
+  d.block0[P.nlev] = nb;
+  hipLaunchKernelGGL(k_diag_inv_levels<T>, dim3(nb), dim3(kBlock), 0, s, P.KE64, d);
+}
+template void launch_diag_inv_levels<double>(Plan&, hipStream_t);
+template void launch_diag_inv_levels<float>(Plan&, hipStream_t);
 
 void launch_coarsen_fixed(const Level& f, const Level& c, hipStream_t s) {
   hipLaunchKernelGGL(k_coarsen_fixed, dim3(nblocks(c.g.nnode)), dim3(kBlock), 0, s, f.g, c.g,

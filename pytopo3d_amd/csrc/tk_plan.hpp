@@ -164,6 +164,11 @@ void launch_stiffness_scale2(Plan& P, const double* rho, double penal, double* s
 template <typename T>
 void launch_coarsen_scale(const Level& f, const Level& c, const T* sf, T* sc, hipStream_t s);
 void launch_coarsen_fixed(const Level& f, const Level& c, hipStream_t s);
+// the element scales of every coarse level, FP64 and FP32 chains together
+void launch_coarsen_scale2(Plan& P, hipStream_t s);
+// the inverse diagonals of every level (precision T) in one launch
+template <typename T>
+void launch_diag_inv_levels(Plan& P, hipStream_t s);
 template <typename T>
 void launch_diag_inv(Plan& P, const Level& L, const T* sl, T* dinv, T* diag_out, hipStream_t s);
 template <typename T>

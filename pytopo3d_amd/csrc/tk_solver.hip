@@ -96,31 +96,28 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const T* __restrict__ 
   if (i < n) z[i] = dinv[i] * r[i];
 }
 
-// F with fixed DOFs zeroed (the reduced system's right-hand side).
+// The reduced system's right-hand side out = F and, when U is given, the
+// initial iterate x = U, both 0 on fixed DOFs.
 __global__ __launch_bounds__(kBlock) void k_mask_rhs(int nnode, const uint8_t* __restrict__ fixed,
                                                       const double* __restrict__ F,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out,
+                                                      const double* __restrict__ U,
+                                                      double* __restrict__ x) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnode) return;
   const uint8_t f = fixed[n];
 #pragma unroll
   for (int c = 0; c < 3; ++c) out[3 * n + c] = ((f >> c) & 1) ? 0.0 : F[3 * n + c];
+  if (U) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[3 * n + c] = ((f >> c) & 1) ? 0.0 : U[3 * n + c];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double* __restrict__ a,
                                                     float* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_zero_fixed(int nnode, const uint8_t* __restrict__ fixed,
-                                                        double* __restrict__ U) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nnode) return;
-  const uint8_t f = fixed[n];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-    if ((f >> c) & 1) U[3 * n + c] = 0.0;
 }
 
 // ---- mixed precision: FP64 flexible CG around an FP32 multigrid correction ----
@@ -622,10 +619,11 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   // factorisation read them); the FP32 copy from the same pass
   if (prec != 0) launch_stiffness_scale2(P, rho, penal, L0.f64.s, L0.f32.s, s);
   else launch_stiffness_scale<double>(P, rho, penal, L0.f64.s, s);
-  for (int l = 0; l + 1 < P.nlev; ++l) {
-    launch_coarsen_scale<double>(P.lv[l], P.lv[l + 1], P.lv[l].f64.s, P.lv[l + 1].f64.s, s);
-    if (prec != 0)
-      launch_coarsen_scale<float>(P.lv[l], P.lv[l + 1], P.lv[l].f32.s, P.lv[l + 1].f32.s, s);
+  if (prec != 0) {
+    launch_coarsen_scale2(P, s);
+  } else {
+    for (int l = 0; l + 1 < P.nlev; ++l)
+      launch_coarsen_scale<double>(P.lv[l], P.lv[l + 1], P.lv[l].f64.s, P.lv[l + 1].f64.s, s);
   }
   // The coarsest factorisation (one workgroup, ~0.4 ms) runs on the plan's
   // set-up stream beside the inverse diagonals, the right-hand side, the
@@ -638,11 +636,8 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   TK_HIP(hipEventRecord(P.cev_out, P.cstream));
   P.coarse_pending = true;
   // inverse diagonals for the V-cycle (and Jacobi) of this precision only
-  for (int l = 0; l < P.nlev; ++l) {
-    Level& L = P.lv[l];
-    if (prec == 0) launch_diag_inv<double>(P, L, L.f64.s, L.f64.dinv, (double*)nullptr, s);
-    else launch_diag_inv<float>(P, L, L.f32.s, L.f32.dinv, (float*)nullptr, s);
-  }
+  if (prec == 0) launch_diag_inv_levels<double>(P, s);
+  else launch_diag_inv_levels<float>(P, s);
   TK_HIP(hipGetLastError());
   P.last_penal = penal;
   P.last_prec = prec;
@@ -662,16 +657,12 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
   }
   P.cnt.solves++;
   double* b = P.lv[0].f64.b;  // masked right-hand side
-  hipLaunchKernelGGL(k_mask_rhs, dim3(nblocks(P.g.nnode)), dim3(kBlock), 0, s, P.g.nnode,
-                     P.lv[0].fixed, F, b);
   // the iterate lives in the plan (x64) so that captured chunks stay valid
-  // for every caller buffer; U is copied in (warm start) and out
+  // for every caller buffer; U is copied in (warm start, masked in the same
+  // pass as the right-hand side) and out
   double* x = P.x64;
-  if (use_x0) {
-    TK_HIP(hipMemcpyAsync(x, U, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_zero_fixed, dim3(nblocks(P.g.nnode)), dim3(kBlock), 0, s, P.g.nnode,
-                       P.lv[0].fixed, x);
-  }
+  hipLaunchKernelGGL(k_mask_rhs, dim3(nblocks(P.g.nnode)), dim3(kBlock), 0, s, P.g.nnode,
+                     P.lv[0].fixed, F, b, use_x0 ? (const double*)U : nullptr, x);
   int total_it = 0;
   double relres = 0.0;
   int rc = (prec == 0) ? pcg<double>(P, b, x, P.r64, P.p64, P.q64, !use_x0, rtol, maxit, jacobi, s)
