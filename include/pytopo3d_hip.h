@@ -71,6 +71,11 @@ typedef struct tk_options {
   double coarse_cheb_lo;  /* low end of the smoothing interval on those levels (0.003):
                              they are latency-bound, so a longer, wider smoother costs
                              little there and brings the cycle closer to a coarse solve */
+  int direction_slots;    /* mixed solves with the element kernel: FP32 search directions
+                             kept (2..64) so that the iterate is updated once per that many
+                             iterations and at the end rather than in every update pass;
+                             0 (default): as many as fit in 2 GiB, at most 64.  The result
+                             does not depend on it (same sums in the same order) */
 } tk_options;
 
 /* Fill *opt with the defaults. */

@@ -45,6 +45,7 @@ class TkOptions(ctypes.Structure):
         ("filter_kind", ctypes.c_int),
         ("coarse_degree", ctypes.c_int),
         ("coarse_cheb_lo", ctypes.c_double),
+        ("direction_slots", ctypes.c_int),
     ]
 
 

@@ -80,7 +80,7 @@ static void free_plan(Plan* P) {
   }
   f(P->coarse_inv64); f(P->coarse_inv32); f(P->coarse_L);
   f(P->r64); f(P->p64); f(P->q64); f(P->x64);
-  f(P->r32); f(P->p32); f(P->q32); f(P->x32);
+  f(P->r32); f(P->p32); f(P->q32); f(P->x32); f(P->pring); f(P->alpha_hist);
   f(P->ework0); f(P->ework1); f(P->ework2); f(P->ework3);
   f(P->partials); f(P->sc);
   if (P->sc_host) (void)hipHostFree(P->sc_host);
@@ -152,6 +152,12 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->filter_kind == 0 || o->filter_kind == 1) P->opt.filter_kind = o->filter_kind;
     if (o->check_every > 0) P->opt.check_every = o->check_every;
     if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
+    if (o->direction_slots != 0 && (o->direction_slots < 2 || o->direction_slots > kRingMax)) {
+      delete P;
+      set_error("direction_slots must be 0 or in [2, " + std::to_string(kRingMax) + "]");
+      return 1;
+    }
+    P->opt.direction_slots = o->direction_slots;
   }
   if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
     delete P;
@@ -240,6 +246,13 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
         dmalloc(&P->x64, g.ndof);
   rc |= dmalloc(&P->r32, g.ndof) | dmalloc(&P->p32, g.ndof) | dmalloc(&P->q32, g.ndof) |
         dmalloc(&P->x32, g.ndof);
+  {
+    const long long slots = kRingBytes / (4LL * g.ndof);
+    P->ring_m = P->opt.direction_slots > 0
+                    ? P->opt.direction_slots
+                    : (int)(slots < 2 ? 2 : (slots > kRingMax ? kRingMax : slots));
+    rc |= dmalloc(&P->pring, (size_t)P->ring_m * g.ndof) | dmalloc(&P->alpha_hist, kRingMax);
+  }
   rc |= dmalloc(&P->ework0, g.nele) | dmalloc(&P->ework1, g.nele) | dmalloc(&P->ework2, g.nele) |
         dmalloc(&P->ework3, g.nele);
   const long long big = g.ndof > g.nele ? g.ndof : g.nele;
@@ -381,6 +394,7 @@ void tk_options_default(tk_options* o) {
   o->filter_kind = d.filter_kind;
   o->coarse_degree = d.coarse_degree;
   o->coarse_cheb_lo = d.coarse_cheb_lo;
+  o->direction_slots = d.direction_slots;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

@@ -54,6 +54,12 @@ struct Epilogue {
   const float* p32o;
   float* p32n;
   const Scalars* sc;
+  // MODE 5 with a direction ring (ring != nullptr): p32n is slot iter % ring_m
+  // and p32o (when p32o != nullptr, used only as that flag) slot
+  // (iter - 1) % ring_m, iter = sc->iter, slots ring_stride floats apart
+  float* ring;
+  long long ring_stride;
+  int ring_m;
 };
 
 // kModeChebZero fuses the first two Chebyshev steps from x = 0: the node
@@ -158,9 +164,15 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   // coefficient (finite whenever z32 is), so the loads stay unconditional.
   float dir_zs = 0.0f, dir_beta = 0.0f;
   const float* __restrict__ dir_p = ep.p32o ? ep.p32o : ep.z32;
+  float* dir_out = ep.p32n;
   if constexpr (MODE == kModeApplyDir) {
     dir_zs = (float)ep.sc->zs;
     dir_beta = ep.p32o ? (float)ep.sc->beta : 0.0f;
+    if (ep.ring) {
+      const int it = ep.sc->iter, m = ep.ring_m;
+      dir_out = ep.ring + (long long)(it % m) * ep.ring_stride;
+      if (ep.p32o) dir_p = ep.ring + (long long)((it + m - 1) % m) * ep.ring_stride;
+    }
   }
   // one explicit rounding order: the owner and the neighbouring tiles (halo
   // node) must form bitwise the same direction, or K p no longer matches the
@@ -417,7 +429,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         if (MODE == kModeApply || MODE == kModeApplyDir) {
           ep.y[i] = val;
           dot += (double)xn * (double)val;
-          if (MODE == kModeApplyDir) ep.p32n[i] = (float)raw_prev[c];
+          if (MODE == kModeApplyDir) dir_out[i] = (float)raw_prev[c];
         } else if (MODE == kModeResidual) {
           // zero on fixed DOFs: the restriction reads no fine mask
           ep.y[i] = fx ? T(0) : eb[c] - val;
@@ -643,6 +655,23 @@ void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, dou
   ep.p32o = p32o;
   ep.p32n = p32n;
   ep.sc = P.sc;
+  launch_el<double, kModeApplyDir>(P, L, L.f64.s, nullptr, ep, 1, &P.sc->done, s);
+}
+
+// The same with the directions in the plan's ring (slots chosen on the device
+// from the iteration counter); have_old = 0: beta = 0 (first direction).
+void el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s) {
+  const Level& L = P.lv[0];
+  P.cnt.applies64++;
+  Epilogue<double> ep{};
+  ep.y = y;
+  ep.z32 = z32;
+  ep.p32o = have_old ? P.pring : nullptr;
+  ep.p32n = nullptr;
+  ep.sc = P.sc;
+  ep.ring = P.pring;
+  ep.ring_stride = P.g.ndof;
+  ep.ring_m = P.ring_m;
   launch_el<double, kModeApplyDir>(P, L, L.f64.s, nullptr, ep, 1, &P.sc->done, s);
 }
 

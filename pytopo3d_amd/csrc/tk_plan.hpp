@@ -40,6 +40,7 @@ struct Options {
   int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node kernel
   int no_graphs = 0;             // 1: no captured solver chunks
   int filter_kind = 0;           // 0: LDS tiles when the radius allows, 1: stencil gather
+  int direction_slots = 0;       // mixed-solver direction ring (0: by memory, else 2..64)
 };
 
 struct Counters {
@@ -47,6 +48,11 @@ struct Counters {
   long long cg_iters = 0, ir_steps = 0, solves = 0;
   long long oc_steps = 0, filter_passes = 0;
 };
+
+// direction ring of the mixed solver: at most kRingMax slots, about
+// kRingBytes in all
+constexpr int kRingMax = 64;
+constexpr long long kRingBytes = 2LL << 30;
 
 struct Plan {
   Grid g{};
@@ -86,6 +92,12 @@ struct Plan {
   // Krylov vectors (fine level)
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *x64 = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  // flexible-CG directions of the last ring_m iterations (FP32, ndof apart) and
+  // their step lengths: the iterate is updated from them every ring_m
+  // iterations and at the end instead of in every update pass
+  float* pring = nullptr;
+  int ring_m = 0;
+  double* alpha_hist = nullptr;
   // element work arrays
   double* ework0 = nullptr;       // OC ratio / design fills
   double* ework1 = nullptr;       // OC candidate
@@ -220,6 +232,7 @@ void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T*
                    double c0, double c1, double c2, int write_d, const int* done, hipStream_t s);
 void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, double* y,
                   hipStream_t s);
+void el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s);
 template <typename T>
 void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const T* sl,
                      const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,

@@ -140,6 +140,10 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __r
 // x += alpha p (p stored in FP32); r -= alpha q; r32 = r / ||r_old|| (the norm
 // before this update; kFinUpdateFlex records it as zs); partials of r.r and of
 // q.z with z = zs z32 the correction p was formed from (the next beta).
+// With a direction ring (ring != nullptr; p32 unused) the iterate is not
+// touched here: alpha goes to alpha_hist[iter % m], and the pass whose slot is
+// m - 1 adds all m ring directions to x, oldest first -- the same sums, in the
+// same order, as one update per iteration.
 __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __restrict__ x,
                                                             double* __restrict__ r,
                                                             const float* __restrict__ p32,
@@ -147,15 +151,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __rest
                                                             float* __restrict__ r32,
                                                             const float* __restrict__ z32,
                                                             const int* __restrict__ done,
-                                                            double* partials, const Scalars* sc) {
+                                                            double* partials, const Scalars* sc,
+                                                            const float* __restrict__ ring, int m,
+                                                            double* __restrict__ alpha_hist) {
   if (*done) return;
   const double alpha = sc->alpha;
   const double rr = sc->rr;
   const double zs = sc->zs;
   const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
+  int slot = 0;
+  if (ring) {
+    slot = sc->iter % m;
+    if (blockIdx.x == 0 && threadIdx.x == 0) alpha_hist[slot] = alpha;
+  }
+  const bool fold = ring && slot == m - 1;
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * (double)p32[i];
+    if (!ring) {
+      x[i] += alpha * (double)p32[i];
+    } else if (fold) {
+      double xi = x[i];
+      for (int j = 0; j < m - 1; ++j) xi += alpha_hist[j] * (double)ring[(long long)j * n + i];
+      x[i] = xi + alpha * (double)ring[(long long)(m - 1) * n + i];
+    }
     const double qi = q[i];
     const double ri = r[i] - alpha * qi;
     r[i] = ri;
@@ -164,6 +182,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __rest
     v[1] += qi * (zs * (double)z32[i]);
   }
   block_partials<2>(v, partials);
+}
+
+// U = x + sum_{j < c} alpha_hist[j] ring_j, in slot order: the ring
+// directions not yet added to the iterate when the solve stopped.
+__global__ __launch_bounds__(kBlock) void k_ring_fold_out(int n, const double* __restrict__ x,
+                                                           const float* __restrict__ ring, int c,
+                                                           const double* __restrict__ alpha_hist,
+                                                           double* __restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i];
+  for (int j = 0; j < c; ++j) xi += alpha_hist[j] * (double)ring[(long long)j * n + i];
+  U[i] = xi;
 }
 
 // z = zs z32; partials of r.z and (unless first) q.z, q = K p_old
@@ -582,8 +613,14 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
   // iteration i writes direction pbuf[i & 1] and reads pbuf[(i - 1) & 1]
   // (none for i = 0); a chunk's kernels and arguments depend on its length
   // and the parity of its first iteration (run_chunks tracks it).
+  // with the element operator the directions go to the plan's ring (the
+  // iterate is updated from it every ring_m iterations and by the caller at
+  // the end, ring_fold_out); else pnew / pold alternate
+  const bool ring = P.use_el && P.pring;
   auto iteration = [&](float* pnew, const float* pold, hipStream_t st) {
-    if (P.use_el) {
+    if (ring) {
+      el_apply_dir_ring(P, z32, pold != nullptr, q, st);
+    } else if (P.use_el) {
       el_apply_dir(P, z32, pold, pnew, q, st);
     } else {
       hipLaunchKernelGGL(k_cg_dir_flex, dim3(nblocks(n)), dim3(kBlock), 0, st, n, z32, pold, pnew,
@@ -591,7 +628,8 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
       launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, st);
     }
     hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, pnew,
-                       q, r32, z32, done, P.partials, P.sc);
+                       q, r32, z32, done, P.partials, P.sc, ring ? P.pring : nullptr, P.ring_m,
+                       P.alpha_hist);
     finalize(P, kFinUpdateFlex, red_blocks(n), st);
     z32 = correction(0, st);
   };
@@ -668,7 +706,12 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
   int rc = (prec == 0) ? pcg<double>(P, b, x, P.r64, P.p64, P.q64, !use_x0, rtol, maxit, jacobi, s)
                        : pcg_mixed(P, b, x, !use_x0, rtol, maxit, jacobi, s);
   if (rc) return rc;
-  TK_HIP(hipMemcpyAsync(U, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  const int c = (prec != 0 && P.use_el && P.pring) ? P.sc_host->iter % P.ring_m : 0;
+  if (c > 0)
+    hipLaunchKernelGGL(k_ring_fold_out, dim3(nblocks(n)), dim3(kBlock), 0, s, n, x, P.pring, c,
+                       P.alpha_hist, U);
+  else
+    TK_HIP(hipMemcpyAsync(U, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   if (prec == 0) {
     total_it = P.sc_host->iter;
     relres = (P.sc_host->bb > 0.0) ? std::sqrt(P.sc_host->rr / P.sc_host->bb) : 0.0;

@@ -200,6 +200,44 @@ def test_solve_true_residual_across_tiles(precision):
     P.close()
 
 
+@pytest.mark.parametrize("solver", ["mg", "jacobi"])
+def test_mixed_direction_slots_give_the_same_solution(solver):
+    """The mixed solver keeps the last `direction_slots` FP32 directions and
+    adds them to the iterate every that many iterations and at the end: the
+    same additions in the same order as one update per iteration, so U and the
+    iteration count are bitwise the same for every slot count (folds during
+    the solve at 2, 3 and 5 slots, only the final one at the default), cold
+    and warm started; and U solves the system."""
+    nelx, nely, nelz = 24, 12, 10
+    note("direction slots", nelx, nely, nelz, solver)
+    rng = np.random.default_rng(31)
+    rd = to_dev(elem_vec(rng.uniform(0.01, 1.0, (nely, nelx, nelz))))
+    Fd = to_dev(oracle.edge_load_force_vector(nelx, nely, nelz))
+    jac = solver == "jacobi"
+    outs = []
+    for slots in (0, 2, 3, 5):
+        P, _ = make_plan(nelx, nely, nelz, 1.5, options={"direction_slots": slots})
+        U = torch.zeros_like(Fd)
+        it, _ = P.solve(rd, 3.0, Fd, U, rtol=1e-9, maxit=20000, precision="mixed", jacobi=jac)
+        U2 = 0.9 * U
+        it2, _ = P.solve(rd, 3.0, Fd, U2, rtol=1e-11, maxit=20000, precision="mixed", jacobi=jac,
+                         warm_start=True, reuse=True)
+        note("  slots", slots, "iters", it, it2)
+        outs.append((it, U.clone(), it2, U2.clone()))
+        if slots == 0:
+            y = torch.empty_like(U2)
+            P.apply_K(rd, 3.0, U2, y)
+            true_rel = float(torch.linalg.vector_norm(Fd - y) / torch.linalg.vector_norm(Fd))
+            assert true_rel <= 1e-10, true_rel
+        P.close()
+    for it, U, it2, U2 in outs[1:]:
+        assert it == outs[0][0] and it2 == outs[0][2]
+        assert torch.equal(U, outs[0][1]) and torch.equal(U2, outs[0][3])
+    for bad in (1, 65):
+        with pytest.raises(ValueError, match="direction_slots"):
+            make_plan(nelx, nely, nelz, 1.5, options={"direction_slots": bad})
+
+
 @pytest.mark.parametrize("ke", ["h8", "asymmetric"])
 @pytest.mark.parametrize("nelx,nely,nelz,rmin", [(6, 4, 3, 1.5), (10, 6, 4, 2.5)])
 def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin, ke):
