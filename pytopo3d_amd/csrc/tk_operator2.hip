@@ -622,7 +622,7 @@ static int el_resident() {
 // Returns the number of workgroups (= partials per reduced quantity).
 template <typename T, int MODE>
 static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
-                     int with_dot, const int* done, hipStream_t s) {
+                     int with_dot, const int* done, hipStream_t s, bool finalize_alpha = true) {
   int nty, zc;
   dim3 grid;
   // the reduction scratch holds 4 values per block of the largest 1-D launch
@@ -637,7 +637,7 @@ static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epi
     hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE>), grid, dim3(256), 0, s, L.g,
                        kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
                        P.partials);
-  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot)
+  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot && finalize_alpha)
     finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
   return (int)(grid.x * grid.y);
 }
@@ -659,8 +659,10 @@ void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, dou
 }
 
 // The same with the directions in the plan's ring (slots chosen on the device
-// from the iteration counter); have_old = 0: beta = 0 (first direction).
-void el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s) {
+// from the iteration counter); have_old = 0: beta = 0 (first direction).  The
+// p.Kp partials are left for the update pass, which forms alpha itself;
+// returns their count.
+int el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s) {
   const Level& L = P.lv[0];
   P.cnt.applies64++;
   Epilogue<double> ep{};
@@ -672,7 +674,7 @@ void el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipSt
   ep.ring = P.pring;
   ep.ring_stride = P.g.ndof;
   ep.ring_m = P.ring_m;
-  launch_el<double, kModeApplyDir>(P, L, L.f64.s, nullptr, ep, 1, &P.sc->done, s);
+  return launch_el<double, kModeApplyDir>(P, L, L.f64.s, nullptr, ep, 1, &P.sc->done, s, false);
 }
 
 template <typename T>

@@ -151,7 +151,9 @@ enum FinOp {
   kFinRatioMax, kFinOcVolume, kFinStats, kFinUpdateFlex, kFinRzPR
 };
 constexpr int kRedBlocks = 2048;   // grid-stride reduction kernels use at most this many blocks
-void finalize(Plan& P, int op, int G, hipStream_t s, double a0 = 0.0, int i0 = 0);
+// partials: nullptr = the plan's reduction scratch
+void finalize(Plan& P, int op, int G, hipStream_t s, double a0 = 0.0, int i0 = 0,
+              const double* partials = nullptr);
 inline int red_blocks(long long n) {
   const int b = nblocks(n);
   return b < kRedBlocks ? (b > 0 ? b : 1) : kRedBlocks;
@@ -232,7 +234,7 @@ void el_cheb_zero2(Plan& P, const Level& L, const T* sl, const T* dinv, const T*
                    double c0, double c1, double c2, int write_d, const int* done, hipStream_t s);
 void el_apply_dir(Plan& P, const float* z32, const float* p32o, float* p32n, double* y,
                   hipStream_t s);
-void el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s);
+int el_apply_dir_ring(Plan& P, const float* z32, int have_old, double* y, hipStream_t s);
 template <typename T>
 void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const T* sl,
                      const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,

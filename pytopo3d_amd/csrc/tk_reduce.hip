@@ -138,9 +138,10 @@ __global__ __launch_bounds__(kFinThreadsMax) void k_finalize(int op, int G,
   }
 }
 
-void finalize(Plan& P, int op, int G, hipStream_t s, double a0, int i0) {
+void finalize(Plan& P, int op, int G, hipStream_t s, double a0, int i0, const double* partials) {
   const int threads = G > 256 * kFinLoads ? kFinThreadsMax : 256;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(threads), 0, s, op, G, P.partials, P.sc, a0, i0);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(threads), 0, s, op, G,
+                     partials ? partials : P.partials, P.sc, a0, i0);
 }
 
 }  // namespace tk

@@ -144,6 +144,10 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_unit(int n, const double* __r
 // touched here: alpha goes to alpha_hist[iter % m], and the pass whose slot is
 // m - 1 adds all m ring directions to x, oldest first -- the same sums, in the
 // same order, as one update per iteration.
+// G_pq > 0: alpha = rz / p.Kp is formed here from the G_pq partials of p.Kp
+// (every block sums them in the same order, so all hold the same alpha; block
+// 0 stores it), in place of a separate finalize launch; the pass's own
+// partials then go to `partials`, which must not overlap pq_partials.
 __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __restrict__ x,
                                                             double* __restrict__ r,
                                                             const float* __restrict__ p32,
@@ -151,11 +155,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_flex(int n, double* __rest
                                                             float* __restrict__ r32,
                                                             const float* __restrict__ z32,
                                                             const int* __restrict__ done,
-                                                            double* partials, const Scalars* sc,
+                                                            double* partials, Scalars* sc,
                                                             const float* __restrict__ ring, int m,
-                                                            double* __restrict__ alpha_hist) {
+                                                            double* __restrict__ alpha_hist,
+                                                            const double* __restrict__ pq_partials,
+                                                            int G_pq) {
   if (*done) return;
-  const double alpha = sc->alpha;
+  double alpha;
+  if (G_pq > 0) {
+    __shared__ double s_pq;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < G_pq; b += blockDim.x) v[0] += pq_partials[b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) s_pq = v[0];
+    __syncthreads();
+    const double pq = s_pq;
+    const bool ok = pq > 0.0;   // else breakdown (or NaN): stop the iteration
+    alpha = ok ? sc->rz / pq : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = alpha;
+      if (!ok) {
+        sc->done = 1;
+        sc->stalled = 1;
+      }
+    }
+    if (!ok) return;
+  } else {
+    alpha = sc->alpha;
+  }
   const double rr = sc->rr;
   const double zs = sc->zs;
   const double inv = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
@@ -617,9 +645,13 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
   // iterate is updated from it every ring_m iterations and by the caller at
   // the end, ring_fold_out); else pnew / pold alternate
   const bool ring = P.use_el && P.pring;
+  // the update pass reads the p.Kp partials at the start of the scratch while
+  // other blocks already write theirs: those go to the upper half
+  double* upd_partials = ring ? P.partials + P.partials_len / 2 : P.partials;
   auto iteration = [&](float* pnew, const float* pold, hipStream_t st) {
+    int G_pq = 0;
     if (ring) {
-      el_apply_dir_ring(P, z32, pold != nullptr, q, st);
+      G_pq = el_apply_dir_ring(P, z32, pold != nullptr, q, st);
     } else if (P.use_el) {
       el_apply_dir(P, z32, pold, pnew, q, st);
     } else {
@@ -628,9 +660,9 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
       launch_apply<double>(P, L0, L0.f64.s, p64, q, nullptr, 1, st);
     }
     hipLaunchKernelGGL(k_cg_update_flex, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, pnew,
-                       q, r32, z32, done, P.partials, P.sc, ring ? P.pring : nullptr, P.ring_m,
-                       P.alpha_hist);
-    finalize(P, kFinUpdateFlex, red_blocks(n), st);
+                       q, r32, z32, done, upd_partials, P.sc, ring ? P.pring : nullptr,
+                       P.ring_m, P.alpha_hist, P.partials, G_pq);
+    finalize(P, kFinUpdateFlex, red_blocks(n), st, 0.0, 0, upd_partials);
     z32 = correction(0, st);
   };
   iteration(pbuf[0], nullptr, s);
