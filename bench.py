@@ -57,6 +57,8 @@ def _parse():
     ap.add_argument("--coarse-degree", type=int, default=0,
                     help="Chebyshev degree on the coarser levels (0: default)")
     ap.add_argument("--cheb-lo", type=float, default=0.0, help="smoothing interval low end (0: default)")
+    ap.add_argument("--coarse-cheb-lo", type=float, default=0.0,
+                    help="smoothing interval low end on the coarser levels (0: default)")
     ap.add_argument("--cheb-hi", type=float, default=0.0,
                     help="smoothing interval high end, fraction of the lambda_max bound (0: default)")
     ap.add_argument("--operator-kind", type=int, default=0,
@@ -85,7 +87,8 @@ def _setup(args):
     # options left at 0 on the command line keep the library defaults (and
     # are reported as such)
     opts = {"operator_kind": args.operator_kind, "smooth_degree": args.smooth_degree,
-            "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo, "cheb_hi": args.cheb_hi}
+            "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo, "cheb_hi": args.cheb_hi,
+            "coarse_cheb_lo": args.coarse_cheb_lo}
     problem = SIMPProblem(nelx, nely, nelz, 3.0, KE, F, fixed, solid, void,
                           solver_options={k: v for k, v in opts.items() if v})
     x0 = np.full((nely, nelx, nelz), 0.3)

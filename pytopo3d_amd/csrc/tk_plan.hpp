@@ -30,7 +30,7 @@ struct Level {
 
 struct Options {
   int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing, two finest levels
-  int coarse_degree = 5;         // ... on the third and coarser levels
+  int coarse_degree = 7;         // ... on the third and coarser levels
   double coarse_cheb_lo = 0.003; // cheb_lo on those levels
   double cheb_lo = 0.06;         // smoothing interval [cheb_lo, cheb_hi] * lambda_max bound
   double cheb_hi = 1.0;
