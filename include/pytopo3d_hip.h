@@ -22,8 +22,11 @@
  *     reference raises RuntimeError, pytopo3d/utils/solver.py:57-58).
  *     tk_last_error() gives the message of the last failure on this thread.
  *   - A plan is not thread-safe; distinct plans are independent.  Calls on a
- *     plan allocate no device memory (all workspaces are made by
- *     tk_plan_create).
+ *     plan allocate no device memory (the workspaces are made by
+ *     tk_plan_create), except the first mixed-precision tk_solve on a plan,
+ *     which allocates the solver's direction ring (direction_slots FP32
+ *     vectors, about 2 GiB by default), and the first tk_solve of each
+ *     precision, which creates its solver stream and graphs.
  */
 #ifndef PYTOPO3D_HIP_H
 #define PYTOPO3D_HIP_H
@@ -46,8 +49,10 @@ enum {
 enum {
   TK_SOLVE_X0 = 1,        /* U holds the initial guess (warm start) */
   TK_SOLVE_JACOBI = 2,    /* Jacobi instead of the multigrid preconditioner */
-  TK_SOLVE_REUSE = 4      /* reuse the hierarchy built by the last solve or tk_prepare (same
-                             rho, penal and precision) */
+  TK_SOLVE_REUSE = 4      /* reuse the hierarchy built by the last solve or tk_prepare: the
+                             same rho buffer (unmodified: the contents are not compared),
+                             penal and precision, and no tk_plan_set_masks since; anything
+                             else is rejected with status 1 */
 };
 
 typedef struct tk_options {
@@ -105,6 +110,13 @@ int tk_plan_set_masks(tk_plan* plan, const uint8_t* elem_flags, const uint8_t* n
 int tk_plan_levels(const tk_plan* plan, int* nlev, int* dims, int cap);
 /* Stencil table (HOST outputs): dy,dx,dz per entry and the weight; returns count in *n. */
 int tk_plan_stencil(const tk_plan* plan, int* offsets, double* weights, int cap, int* n);
+/* Density-filter kernel the plan uses (HOST outputs, each may be NULL):
+ * tiled = 1 for the LDS-tiled stencil (stencil reach <= 3 elements per axis,
+ * filter_kind 0), 0 for the stencil gather; halo = the stencil's largest
+ * per-axis offset, which is the tile halo of the tiled kernel; nsten = stencil
+ * entries.  Not in the reference (its filter is a sparse matrix,
+ * pytopo3d/utils/filter.py:122-228); it lets tests name the kernel variant. */
+int tk_plan_filter_info(const tk_plan* plan, int* tiled, int* halo, int* nsten);
 /* Copy the filter row sums Hs (device output, nele doubles). */
 int tk_filter_row_sums(tk_plan* plan, double* Hs, void* stream);
 

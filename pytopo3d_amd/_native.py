@@ -23,8 +23,10 @@ TK_SOLVE_X0 = 1
 TK_SOLVE_JACOBI = 2
 TK_SOLVE_REUSE = 4
 
-PRECISIONS = {"fp64": TK_FP64, "mixed": TK_MIXED_FP32, "fp32": TK_MIXED_FP32,
-              "mixed_fp32": TK_MIXED_FP32}
+# "mixed": FP64 flexible CG around an FP32 multigrid correction.  There is no
+# all-FP32 solve: the FP64 outer iteration is what reaches FP64 accuracy at the
+# 1e-9 SIMP contrast.
+PRECISIONS = {"fp64": TK_FP64, "mixed": TK_MIXED_FP32, "mixed_fp32": TK_MIXED_FP32}
 
 
 class SolveNotConverged(RuntimeError):
@@ -60,6 +62,7 @@ _SIGS = {
     "tk_plan_set_masks": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_longlong), _P]),
     "tk_plan_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
     "tk_plan_stencil": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D), _I, ctypes.POINTER(_I)]),
+    "tk_plan_filter_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -190,6 +193,13 @@ class Plan:
                                      w.ctypes.data_as(ctypes.POINTER(_D)), n.value,
                                      ctypes.byref(n)))
         return off, w   # columns (dy, dx, dz)
+
+    def filter_info(self) -> dict:
+        """Filter kernel of the plan: {"tiled": bool, "halo": int, "nsten": int}."""
+        t, h, n = _I(), _I(), _I()
+        _check(lib().tk_plan_filter_info(self._h, ctypes.byref(t), ctypes.byref(h),
+                                         ctypes.byref(n)))
+        return {"tiled": bool(t.value), "halo": int(h.value), "nsten": int(n.value)}
 
     def row_sums(self, out: torch.Tensor) -> torch.Tensor:
         _check(lib().tk_filter_row_sums(self._h, _ptr(out), _stream()))

@@ -202,6 +202,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
         if (wt > 1e-9) {
           off.push_back((dy + 16) | ((dx + 16) << 8) | ((dz + 16) << 16));
           w.push_back(wt);
+          P->reach = std::max(P->reach, std::max(std::abs(dy), std::max(std::abs(dx), std::abs(dz))));
         }
       }
   if ((int)off.size() > kMaxStencil) {
@@ -247,11 +248,12 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   rc |= dmalloc(&P->r32, g.ndof) | dmalloc(&P->p32, g.ndof) | dmalloc(&P->q32, g.ndof) |
         dmalloc(&P->x32, g.ndof);
   {
+    // slot count of the mixed solver's direction ring (allocated by its first
+    // mixed solve)
     const long long slots = kRingBytes / (4LL * g.ndof);
     P->ring_m = P->opt.direction_slots > 0
                     ? P->opt.direction_slots
                     : (int)(slots < 2 ? 2 : (slots > kRingMax ? kRingMax : slots));
-    rc |= dmalloc(&P->pring, (size_t)P->ring_m * g.ndof) | dmalloc(&P->alpha_hist, kRingMax);
   }
   rc |= dmalloc(&P->ework0, g.nele) | dmalloc(&P->ework1, g.nele) | dmalloc(&P->ework2, g.nele) |
         dmalloc(&P->ework3, g.nele);
@@ -429,6 +431,7 @@ int tk_plan_set_masks(tk_plan* plan, const uint8_t* elem_flags, const uint8_t* n
     set_error("elem_flags and node_fixed are required");
     return 1;
   }
+  P.mask_gen++;   // invalidates the set-up a TK_SOLVE_REUSE would use
   TK_HIP(hipMemcpyAsync(P.eflags, elem_flags, P.g.nele, hipMemcpyDeviceToDevice, s));
   TK_HIP(hipMemcpyAsync(P.lv[0].fixed, node_fixed, P.g.nnode, hipMemcpyDeviceToDevice, s));
   for (int l = 0; l + 1 < P.nlev; ++l) tk::launch_coarsen_fixed(P.lv[l], P.lv[l + 1], s);
@@ -479,6 +482,18 @@ int tk_plan_stencil(const tk_plan* plan, int* offsets, double* weights, int cap,
     }
     if (weights) weights[k] = w[k];
   }
+  return 0;
+}
+
+int tk_plan_filter_info(const tk_plan* plan, int* tiled, int* halo, int* nsten) {
+  if (!plan || !plan->p) {
+    set_error("null plan");
+    return 1;
+  }
+  const tk::Plan& P = *plan->p;
+  if (tiled) *tiled = P.ftiled ? 1 : 0;
+  if (halo) *halo = P.ftiled ? P.fhalo : P.reach;
+  if (nsten) *nsten = P.nsten;
   return 0;
 }
 
@@ -612,9 +627,10 @@ int tk_solve(tk_plan* plan, int prec, const double* rho, double penal, const dou
   }
   // each set-up builds the state of its own precision only (a mixed set-up
   // forms no FP64 inverse diagonals)
-  if ((flags & TK_SOLVE_REUSE) && (P.last_penal != penal || prec != P.last_prec)) {
-    set_error("TK_SOLVE_REUSE needs a previous solve or tk_prepare on this plan with the same rho, "
-              "penal and precision");
+  if ((flags & TK_SOLVE_REUSE) && (P.last_penal != penal || prec != P.last_prec ||
+                                    rho != P.last_rho || P.setup_mask_gen != P.mask_gen)) {
+    set_error("TK_SOLVE_REUSE needs a previous solve or tk_prepare on this plan with the same rho "
+              "buffer, penal and precision, and no tk_plan_set_masks since");
     return 1;
   }
   return tk::solve(P, prec, rho, penal, F, U, rtol, maxit, flags, iters_out, relres_out,

@@ -72,6 +72,7 @@ struct Plan {
   // density filter
   double rmin = 0.0;
   int nsten = 0, R = 0;
+  int reach = 0;                  // largest per-axis stencil offset
   int* sten_off = nullptr;        // packed (dy+16) | (dx+16)<<8 | (dz+16)<<16
   double* sten_w = nullptr;
   double* Hs = nullptr;
@@ -97,6 +98,7 @@ struct Plan {
   // iterations and at the end instead of in every update pass
   float* pring = nullptr;
   int ring_m = 0;
+  bool ring_tried = false;        // allocated by the first mixed solve (ensure_ring)
   double* alpha_hist = nullptr;
   // element work arrays
   double* ework0 = nullptr;       // OC ratio / design fills
@@ -110,9 +112,12 @@ struct Plan {
   Scalars* sc_host = nullptr;     // pinned mirror
   Options opt;
   Counters cnt;
-  // bookkeeping of the last solve
+  // bookkeeping of the last set-up (TK_SOLVE_REUSE must match all of it)
   double last_penal = -1.0;
   int last_prec = 0;
+  const double* last_rho = nullptr;
+  long long mask_gen = 0;         // bumped by every tk_plan_set_masks
+  long long setup_mask_gen = -1;  // mask_gen at the last set-up
   // captured Krylov chunks, [precision][jacobi]: every launch argument of a
   // chunk is plan-owned (the iterate lives in x64), so a chunk is captured
   // once per plan and replayed on gstream.

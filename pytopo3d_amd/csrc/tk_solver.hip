@@ -711,6 +711,28 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   TK_HIP(hipGetLastError());
   P.last_penal = penal;
   P.last_prec = prec;
+  P.last_rho = rho;
+  P.setup_mask_gen = P.mask_gen;
+  return 0;
+}
+
+// The mixed solver's direction ring (ring_m FP32 vectors and their step
+// lengths) is allocated by the first mixed solve with the element operator,
+// before any mixed chunk is captured, so FP64-only plans never hold it.
+static int ensure_ring(Plan& P) {
+  if (P.ring_tried || !P.use_el) return 0;
+  P.ring_tried = true;
+  const long long n = P.g.ndof;
+  if (hipMalloc((void**)&P.pring, sizeof(float) * (size_t)P.ring_m * (size_t)n) != hipSuccess ||
+      hipMalloc((void**)&P.alpha_hist, sizeof(double) * kRingMax) != hipSuccess) {
+    (void)hipGetLastError();
+    if (P.pring) (void)hipFree(P.pring);
+    P.pring = nullptr;
+    set_error("could not allocate the mixed solver's direction ring (" +
+              std::to_string(P.ring_m) + " FP32 vectors of " + std::to_string(n) +
+              " entries); lower direction_slots");
+    return 2;
+  }
   return 0;
 }
 
@@ -721,6 +743,9 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
           double rtol, int maxit, int flags, int* iters_out, double* relres_out, hipStream_t s) {
   const int n = P.g.ndof;
   const int use_x0 = flags & 1, jacobi = (flags >> 1) & 1, reuse = (flags >> 2) & 1;
+  if (prec != 0) {
+    if (int rc = ensure_ring(P)) return rc;
+  }
   if (!reuse) {
     int rc = setup_hierarchy(P, rho, penal, prec, s);
     if (rc) return rc;

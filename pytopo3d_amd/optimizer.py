@@ -18,6 +18,11 @@ Keywords beyond the reference's (all optional):
   solver_maxit   Krylov iteration cap per solve
   solver_options dict of tk_options fields (smoothing degree, levels, ...)
   timings        dict filled with per-phase wall times and work counters
+  iteration_callback
+                 called after every design iteration with a dict: beta, iter,
+                 total, c, change, vol, gray, and the device tensors x (the
+                 new design) and rho (the physical density the iteration
+                 solved with), element order; valid until the next iteration
 """
 
 from __future__ import annotations
@@ -114,13 +119,59 @@ class SIMPProblem:
 
 
 class DesignState:
-    """Device buffers of one design trajectory (element order)."""
+    """Device buffers of one design trajectory (element order).
+
+    ``rho`` and ``rho_alt`` alternate: a design step re-filters the new design
+    into ``rho_alt`` and swaps them, so after a step ``rho_alt`` still holds the
+    physical density the step solved with (the reference's ``rho_fea``), ready
+    for a history snapshot without a device copy.
+    """
 
     def __init__(self, problem: SIMPProblem, x0: np.ndarray):
         self.x = _to_dev_elem(x0, problem.dev)
         self.x_new = problem.new_elem()
         self.rf, self.rho, self.drho = problem.new_elem(), problem.new_elem(), problem.new_elem()
+        self.rho_alt = problem.new_elem()
         self.dc_dx, self.dv_dx = problem.new_elem(), problem.new_elem()
+        self.snapshots: Optional["HistorySnapshots"] = None
+
+
+class HistorySnapshots:
+    """Asynchronous device-to-host copies of density snapshots.
+
+    Each snapshot goes to its own pinned host buffer on a side stream that
+    starts after the work queued so far on the caller's stream; the caller's
+    stream waits for the copy only before it overwrites the source buffer
+    (``fence``), so the copy overlaps the next linear solve.  ``arrays``
+    synchronises and returns the snapshots as ``(nely, nelx, nelz)`` float64
+    arrays (the history format of pytopo3d/core/optimizer.py:661-669).
+    """
+
+    def __init__(self, shape):
+        self.shape = shape
+        self.side = torch.cuda.Stream()
+        self.done = None
+        self.host = []
+
+    def take(self, src: torch.Tensor) -> None:
+        ready = torch.cuda.Event()
+        ready.record()
+        dst = torch.empty(src.numel(), dtype=src.dtype, pin_memory=True)
+        with torch.cuda.stream(self.side):
+            self.side.wait_event(ready)
+            dst.copy_(src, non_blocking=True)
+            self.done = torch.cuda.Event()
+            self.done.record()
+        self.host.append(dst)
+
+    def fence(self) -> None:
+        if self.done is not None:
+            torch.cuda.current_stream().wait_event(self.done)
+            self.done = None
+
+    def arrays(self):
+        self.side.synchronize()
+        return [t.numpy().reshape(self.shape, order="F") for t in self.host]
 
 
 def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float, eta: float,
@@ -136,11 +187,14 @@ def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float
     c = problem.sensitivities(U, st.rho, st.drho, penal, dc_dx=st.dc_dx, dv_dx=st.dv_dx)
     t2 = time.perf_counter()
     plan.oc_update(st.x, st.dc_dx, st.dv_dx, move, target, beta, eta, st.x_new)
-    plan.filter_project(st.x_new, beta, eta, st.rf, st.rho, st.drho)
-    plan.design_stats(st.rho, st.x, st.x_new)
+    if st.snapshots is not None:
+        st.snapshots.fence()   # rho_alt may still be copied to the host
+    plan.filter_project(st.x_new, beta, eta, st.rf, st.rho_alt, st.drho)
+    plan.design_stats(st.rho_alt, st.x, st.x_new)
     res = plan.results()
     t3 = time.perf_counter()
     st.x, st.x_new = st.x_new, st.x
+    st.rho, st.rho_alt = st.rho_alt, st.rho
     return {"c": c, "change": float(res["change"]), "vol": res["vol_sum"] / problem.nfree,
             "gray": res["gray_cnt"] / problem.nfree, "oc_steps": int(res["oc_steps"]),
             "oc_evals": int(res["oc_evals"]),
@@ -271,6 +325,7 @@ def top3d(
     solver_options: Optional[dict] = None,
     final_evaluation: bool = True,
     timings: Optional[dict] = None,
+    iteration_callback=None,
 ):
     """Density-filtered SIMP with Heaviside continuation on the GPU.
 
@@ -318,155 +373,166 @@ def top3d(
     t_setup = time.perf_counter()
     problem = _make_problem(nelx, nely, nelz, rmin, material_params, elem_size, force_field,
                             support_mask, solid, void, force_vector, solver_options)
-    plan = problem.plan
-    dev = problem.dev
-    logger.info("Linear solver: matrix-free H8 %s-PCG (%s) on %s", solver, precision,
-                torch.cuda.get_device_name(dev))
-    history = ({"density_history": [], "iteration_history": [], "compliance_history": [],
-                "beta_history": []} if save_history else None)
-    solve_kw = dict(precision=precision, rtol=solver_rtol, maxit=solver_maxit, solver=solver)
+    try:
+        plan = problem.plan
+        dev = problem.dev
+        logger.info("Linear solver: matrix-free H8 %s-PCG (%s) on %s", solver, precision,
+                    torch.cuda.get_device_name(dev))
+        history = ({"density_history": [], "iteration_history": [], "compliance_history": [],
+                    "beta_history": []} if save_history else None)
+        solve_kw = dict(precision=precision, rtol=solver_rtol, maxit=solver_maxit, solver=solver)
 
-    x0 = np.full(expected_shape, float(volfrac))
-    x0[solid] = 1.0
-    x0[void] = 0.0
-    st = DesignState(problem, x0)
-    t_setup = time.perf_counter() - t_setup
+        x0 = np.full(expected_shape, float(volfrac))
+        x0[solid] = 1.0
+        x0[void] = 0.0
+        st = DesignState(problem, x0)
+        if save_history:
+            st.snapshots = HistorySnapshots(expected_shape)
+        t_setup = time.perf_counter() - t_setup
 
-    tm = {"setup_s": t_setup, "solve_s": 0.0, "sens_s": 0.0, "oc_s": 0.0, "iter_s": 0.0,
-          "solve_iters": [], "oc_steps": []}
-    total_loop = 0
-    c = float("nan")
-    stage_summaries = []
-    eta = projection_eta
-    for beta in beta_schedule:
-        logger.info("Starting projection stage: beta=%g", beta)
-        change = float("inf")
-        stage_loop = 0
-        c_prev = float("nan")
-        plan.filter_project(st.x, beta, eta, st.rf, st.rho, st.drho)
-        vmin, vmax = plan.volume_bounds(beta, eta)
-        target_within = vmin - 1.0e-10 <= volfrac <= vmax + 1.0e-10
-        if volfrac < vmin - 1.0e-10:
-            problem.close()
-            raise ValueError(
-                "projected free-volume target is infeasible at "
-                f"beta={beta:g}: target={volfrac:.6f}, minimum achievable="
-                f"{vmin:.6f}. Protected-solid density "
-                "spreads into the free region through the density filter.")
-        if volfrac > vmax + 1.0e-10:
-            logger.warning(
-                "Projected free-volume target %.6f exceeds the maximum "
-                "achievable %.6f at beta=%g; the volume upper bound is "
-                "inactive at the all-solid free design.", volfrac, vmax, beta)
-        vol_frac = float("nan")
-        gray = float("nan")
-        while change > tolx and stage_loop < maxloop:
-            stage_loop += 1
-            total_loop += 1
-            t0 = time.perf_counter()
-            snapshot = save_history and ((total_loop - 1) % history_frequency == 0
-                                         or stage_loop == maxloop)
-            rho_fea = st.rho.clone() if save_history else None
-            r = design_step(problem, st, penal, beta, eta, move, target_free_volume, solve_kw)
-            c = r["c"]
-            change, vol_frac, gray = r["change"], r["vol"], r["gray"]
-            tm["solve_s"] += r["t_solve"]
-            tm["sens_s"] += r["t_sens"]
-            tm["oc_s"] += r["t_oc"]
-            tm["solve_iters"].append(r["solve_iters"])
-            tm["oc_steps"].append(r["oc_steps"])
-            c_delta = c - c_prev if np.isfinite(c_prev) else float("nan")
-            c_prev = c
-            elapsed = time.perf_counter() - t0
-            tm["iter_s"] += elapsed
-            if save_history and (snapshot or change <= tolx):
-                history["density_history"].append(_to_host_elem(rho_fea, expected_shape))
-                history["iteration_history"].append(total_loop - 1)
-                history["compliance_history"].append(c)
-                history["beta_history"].append(beta)
-            logger.info(
-                "beta=%4.1f iter=%4d total=%4d C=%.6e dC=%.3e "
-                "vol=%.5f gray=%.5f change=%.5e time=%.2fs",
-                beta, stage_loop, total_loop, c, c_delta, vol_frac, gray, change, elapsed)
-        stage_converged = change <= tolx
-        if not stage_converged:
-            logger.warning(
-                "Projection stage beta=%g reached maxloop=%d without "
-                "converging: final change %.6e exceeds tolx %.6e.", beta, maxloop, change, tolx)
-        stage_summaries.append({
-            "beta": float(beta),
-            "iterations": int(stage_loop),
-            "converged": bool(stage_converged),
-            "physical_density_fraction": float(vol_frac),
-            "volume_fraction_error": float(vol_frac) - float(volfrac),
-            "gray_fraction_005_095": float(gray),
-            "final_change": float(change),
-            "minimum_achievable_physical_fraction": vmin,
-            "maximum_achievable_physical_fraction": vmax,
-            "target_within_achievable_range": target_within,
-        })
+        tm = {"setup_s": t_setup, "solve_s": 0.0, "sens_s": 0.0, "oc_s": 0.0, "iter_s": 0.0,
+              "solve_iters": [], "oc_steps": []}
+        total_loop = 0
+        c = float("nan")
+        stage_summaries = []
+        eta = projection_eta
+        for beta in beta_schedule:
+            logger.info("Starting projection stage: beta=%g", beta)
+            change = float("inf")
+            stage_loop = 0
+            c_prev = float("nan")
+            plan.filter_project(st.x, beta, eta, st.rf, st.rho, st.drho)
+            vmin, vmax = plan.volume_bounds(beta, eta)
+            target_within = vmin - 1.0e-10 <= volfrac <= vmax + 1.0e-10
+            if volfrac < vmin - 1.0e-10:
+                raise ValueError(
+                    "projected free-volume target is infeasible at "
+                    f"beta={beta:g}: target={volfrac:.6f}, minimum achievable="
+                    f"{vmin:.6f}. Protected-solid density "
+                    "spreads into the free region through the density filter.")
+            if volfrac > vmax + 1.0e-10:
+                logger.warning(
+                    "Projected free-volume target %.6f exceeds the maximum "
+                    "achievable %.6f at beta=%g; the volume upper bound is "
+                    "inactive at the all-solid free design.", volfrac, vmax, beta)
+            vol_frac = float("nan")
+            gray = float("nan")
+            while change > tolx and stage_loop < maxloop:
+                stage_loop += 1
+                total_loop += 1
+                t0 = time.perf_counter()
+                snapshot = save_history and ((total_loop - 1) % history_frequency == 0
+                                             or stage_loop == maxloop)
+                r = design_step(problem, st, penal, beta, eta, move, target_free_volume, solve_kw)
+                c = r["c"]
+                change, vol_frac, gray = r["change"], r["vol"], r["gray"]
+                tm["solve_s"] += r["t_solve"]
+                tm["sens_s"] += r["t_sens"]
+                tm["oc_s"] += r["t_oc"]
+                tm["solve_iters"].append(r["solve_iters"])
+                tm["oc_steps"].append(r["oc_steps"])
+                if iteration_callback is not None:
+                    iteration_callback({"beta": beta, "iter": stage_loop, "total": total_loop,
+                                        "c": c, "change": change, "vol": vol_frac, "gray": gray,
+                                        "x": st.x, "rho": st.rho_alt})
+                c_delta = c - c_prev if np.isfinite(c_prev) else float("nan")
+                c_prev = c
+                elapsed = time.perf_counter() - t0
+                tm["iter_s"] += elapsed
+                if save_history and (snapshot or change <= tolx):
+                    # st.rho_alt holds the density this step solved with; its
+                    # copy to pinned host memory overlaps the next solve
+                    st.snapshots.take(st.rho_alt)
+                    history["density_history"].append(len(st.snapshots.host) - 1)
+                    history["iteration_history"].append(total_loop - 1)
+                    history["compliance_history"].append(c)
+                    history["beta_history"].append(beta)
+                logger.info(
+                    "beta=%4.1f iter=%4d total=%4d C=%.6e dC=%.3e "
+                    "vol=%.5f gray=%.5f change=%.5e time=%.2fs",
+                    beta, stage_loop, total_loop, c, c_delta, vol_frac, gray, change, elapsed)
+            stage_converged = change <= tolx
+            if not stage_converged:
+                logger.warning(
+                    "Projection stage beta=%g reached maxloop=%d without "
+                    "converging: final change %.6e exceeds tolx %.6e.", beta, maxloop, change, tolx)
+            stage_summaries.append({
+                "beta": float(beta),
+                "iterations": int(stage_loop),
+                "converged": bool(stage_converged),
+                "physical_density_fraction": float(vol_frac),
+                "volume_fraction_error": float(vol_frac) - float(volfrac),
+                "gray_fraction_005_095": float(gray),
+                "final_change": float(change),
+                "minimum_achievable_physical_fraction": vmin,
+                "maximum_achievable_physical_fraction": vmax,
+                "target_within_achievable_range": target_within,
+            })
 
-    final_beta = beta_schedule[-1]
-    plan.filter_project(st.x, final_beta, eta, st.rf, st.rho, st.drho)
-    final_x = _to_host_elem(st.x, expected_shape)
-    final_rf = _to_host_elem(st.rf, expected_shape)
-    final_xPhys = _to_host_elem(st.rho, expected_shape)
-    if not final_evaluation:
-        problem.close()
+        if save_history:
+            snaps = st.snapshots.arrays()
+            history["density_history"] = [snaps[i] for i in history["density_history"]]
+        final_beta = beta_schedule[-1]
+        plan.filter_project(st.x, final_beta, eta, st.rf, st.rho, st.drho)
+        final_x = _to_host_elem(st.x, expected_shape)
+        final_rf = _to_host_elem(st.rf, expected_shape)
+        final_xPhys = _to_host_elem(st.rho, expected_shape)
+        if not final_evaluation:
+            if timings is not None:
+                timings.update(tm, counters=plan.counters())
+            return final_xPhys, history, float("nan"), None
+
+        ts = time.perf_counter()
+        final_compliance, _ = _fixed_geometry(problem, final_xPhys, penal, precision, solver_rtol,
+                                              solver_maxit, solver)
+        if save_history:
+            history["density_history"].append(final_xPhys.copy())
+            history["iteration_history"].append(total_loop)
+            history["compliance_history"].append(float(final_compliance))
+            history["beta_history"].append(float(final_beta))
+        rho_binary = (final_xPhys >= 0.5).astype(float)
+        rho_binary[solid] = 1.0
+        rho_binary[void] = 0.0
+        binary_compliance, _ = _fixed_geometry(problem, rho_binary, penal, precision, solver_rtol,
+                                               solver_maxit, solver)
+        tm["final_eval_s"] = time.perf_counter() - ts
+        binary_delta = ((binary_compliance - final_compliance) / final_compliance
+                        if final_compliance != 0.0 else None)
+        free_values = final_xPhys[free]
+        gray_mask = (free_values > 0.05) & (free_values < 0.95)
+        projection_metrics = {
+            "projection_enabled": True,
+            "projection_beta": float(final_beta),
+            "projection_beta_schedule": [float(b) for b in beta_schedule],
+            "projection_eta": float(projection_eta),
+            "penal": float(penal),
+            "design_variable_fraction": float(np.mean(final_x[free])),
+            "filtered_density_fraction": float(np.mean(final_rf[free])),
+            "physical_density_fraction": float(np.mean(free_values)),
+            "gray_fraction_005_095": float(np.mean(gray_mask.astype(float))),
+            "projected_compliance": float(final_compliance),
+            "binary_compliance": float(binary_compliance),
+            "binary_compliance_delta": None if binary_delta is None else float(binary_delta),
+            "binary_threshold": 0.5,
+            "target_free_volume_fraction": float(volfrac),
+            "physical_volume_fraction_error": float(np.mean(free_values) - volfrac),
+            "projection_total_iterations": int(total_loop),
+            "projection_stage_summaries": stage_summaries,
+            "projection_converged": all(s["converged"] for s in stage_summaries),
+            "projection_target_within_achievable_range": all(
+                s["target_within_achievable_range"] for s in stage_summaries),
+        }
+        logger.info(
+            "Final projection: beta=%.1f eta=%.3f vol=%.5f gray=%.5f "
+            "C_projected=%.6e C_binary=%.6e binary_delta=%s", final_beta, projection_eta,
+            projection_metrics["physical_density_fraction"], projection_metrics["gray_fraction_005_095"],
+            final_compliance, binary_compliance,
+            "n/a" if binary_delta is None else f"{binary_delta:.6e}")
+        if diagnostics_out is not None:
+            diagnostics_out.clear()
+            diagnostics_out.update(projection_metrics)
         if timings is not None:
             timings.update(tm, counters=plan.counters())
-        return final_xPhys, history, float("nan"), None
-
-    ts = time.perf_counter()
-    final_compliance, _ = _fixed_geometry(problem, final_xPhys, penal, precision, solver_rtol,
-                                          solver_maxit, solver)
-    if save_history:
-        history["density_history"].append(final_xPhys.copy())
-        history["iteration_history"].append(total_loop)
-        history["compliance_history"].append(float(final_compliance))
-        history["beta_history"].append(float(final_beta))
-    rho_binary = (final_xPhys >= 0.5).astype(float)
-    rho_binary[solid] = 1.0
-    rho_binary[void] = 0.0
-    binary_compliance, _ = _fixed_geometry(problem, rho_binary, penal, precision, solver_rtol,
-                                           solver_maxit, solver)
-    tm["final_eval_s"] = time.perf_counter() - ts
-    binary_delta = ((binary_compliance - final_compliance) / final_compliance
-                    if final_compliance != 0.0 else None)
-    free_values = final_xPhys[free]
-    gray_mask = (free_values > 0.05) & (free_values < 0.95)
-    projection_metrics = {
-        "projection_enabled": True,
-        "projection_beta": float(final_beta),
-        "projection_beta_schedule": [float(b) for b in beta_schedule],
-        "projection_eta": float(projection_eta),
-        "penal": float(penal),
-        "design_variable_fraction": float(np.mean(final_x[free])),
-        "filtered_density_fraction": float(np.mean(final_rf[free])),
-        "physical_density_fraction": float(np.mean(free_values)),
-        "gray_fraction_005_095": float(np.mean(gray_mask.astype(float))),
-        "projected_compliance": float(final_compliance),
-        "binary_compliance": float(binary_compliance),
-        "binary_compliance_delta": None if binary_delta is None else float(binary_delta),
-        "binary_threshold": 0.5,
-        "target_free_volume_fraction": float(volfrac),
-        "physical_volume_fraction_error": float(np.mean(free_values) - volfrac),
-        "projection_total_iterations": int(total_loop),
-        "projection_stage_summaries": stage_summaries,
-        "projection_converged": all(s["converged"] for s in stage_summaries),
-        "projection_target_within_achievable_range": all(
-            s["target_within_achievable_range"] for s in stage_summaries),
-    }
-    logger.info(
-        "Final projection: beta=%.1f eta=%.3f vol=%.5f gray=%.5f "
-        "C_projected=%.6e C_binary=%.6e binary_delta=%s", final_beta, projection_eta,
-        projection_metrics["physical_density_fraction"], projection_metrics["gray_fraction_005_095"],
-        final_compliance, binary_compliance,
-        "n/a" if binary_delta is None else f"{binary_delta:.6e}")
-    if diagnostics_out is not None:
-        diagnostics_out.clear()
-        diagnostics_out.update(projection_metrics)
-    if timings is not None:
-        timings.update(tm, counters=plan.counters())
-    problem.close()
-    return final_xPhys, history, float(final_compliance), None
+        return final_xPhys, history, float(final_compliance), None
+    finally:
+        problem.close()

@@ -18,6 +18,9 @@ pytestmark = pytest.mark.gpu
 
 GRIDS = [(3, 2, 4, 1.6), (6, 4, 3, 1.5), (7, 5, 4, 2.5), (9, 6, 5, 3.0), (12, 5, 7, 2.0),
          (37, 66, 21, 2.4)]
+# stencil reach 3 (rmin in (3, 4]): the tiled kernel's halo-3 instantiation;
+# the second grid spans several tiles in every direction (tile 32 x 16 x 8)
+HALO3 = [(9, 7, 6, 3.5), (20, 12, 10, 4.0), (70, 40, 19, 4.0)]
 
 
 def _stencil_matrix(P, nelx, nely, nelz):
@@ -53,8 +56,26 @@ def test_stencil_and_row_sums_match_oracle(nelx, nely, nelz, rmin):
 FILTER_KINDS = [0, 1]   # LDS tiles, stencil gather
 
 
+@pytest.mark.parametrize("nelx,nely,nelz,rmin,halo,nsten", [(9, 6, 5, 3.0, 2, 93),
+                                                           (9, 7, 6, 3.5, 3, 179),
+                                                           (20, 12, 10, 4.0, 3, 251),
+                                                           (6, 4, 3, 1.5, 1, 19),
+                                                           (8, 8, 8, 5.0, 4, 485)])
+def test_filter_kernel_variant_is_the_one_intended(nelx, nely, nelz, rmin, halo, nsten):
+    """The plan takes the LDS-tiled kernel with the stencil's halo up to a
+    reach of 3 (rmin <= 4) and the gather kernel beyond it or when asked, so
+    the parity grids above exercise each instantiation they are meant to."""
+    note("filter variant", rmin)
+    for fk in FILTER_KINDS:
+        P, _ = make_plan(nelx, nely, nelz, rmin, options={"filter_kind": fk})
+        info = P.filter_info()
+        assert info["halo"] == halo and info["nsten"] == nsten
+        assert info["tiled"] == (fk == 0 and halo <= 3)
+        P.close()
+
+
 @pytest.mark.parametrize("fk", FILTER_KINDS)
-@pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS)
+@pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS + HALO3)
 @pytest.mark.parametrize("beta", [1.0, 8.0])
 def test_filter_project_matches_oracle(nelx, nely, nelz, rmin, beta, fk):
     note("filter_project", nelx, nely, nelz, rmin, beta, "filter_kind", fk)
@@ -81,7 +102,7 @@ def test_filter_project_matches_oracle(nelx, nely, nelz, rmin, beta, fk):
 
 
 @pytest.mark.parametrize("fk", FILTER_KINDS)
-@pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS)
+@pytest.mark.parametrize("nelx,nely,nelz,rmin", GRIDS + HALO3)
 def test_filter_transpose_matches_oracle_chain_rule(nelx, nely, nelz, rmin, fk):
     note("transpose2", nelx, nely, nelz, rmin, "filter_kind", fk)
     rng = np.random.default_rng(7 + nelz)
@@ -279,7 +300,8 @@ def test_energy_and_sensitivities_match_oracle(nelx, nely, nelz, rmin, ke):
 @pytest.mark.parametrize("fk", FILTER_KINDS)
 @pytest.mark.parametrize("scale", [1.0, 1e-18, 1e18])
 @pytest.mark.parametrize("nelx,nely,nelz,rmin,beta", [(6, 4, 3, 1.5, 1.0), (10, 6, 4, 2.5, 4.0),
-                                                      (12, 8, 6, 3.0, 8.0), (40, 20, 35, 3.0, 2.0)])
+                                                      (12, 8, 6, 3.0, 8.0), (40, 20, 35, 3.0, 2.0),
+                                                      (9, 7, 6, 3.5, 2.0), (20, 12, 10, 4.0, 8.0)])
 def test_oc_update_matches_oracle(nelx, nely, nelz, rmin, beta, scale, fk):
     note("oc", nelx, nely, nelz, rmin, beta, scale, "filter_kind", fk)
     rng = np.random.default_rng(17)

@@ -237,7 +237,7 @@ def test_autograd_gradient_with_protected_regions_matches_oracle_and_fd():
     c_o = float(np.sum(oracle.stiffness_scale(rho, 3.0) * ce))
     g_o = elem_vec(oracle.filter_chain_rule(-3.0 * (1 - 1e-9) * rho ** 2.0 * ce, d, H, Hs))
     assert c.item() == pytest.approx(c_o, rel=1e-10)
-    assert np.abs(g - g_o).max() <= 1e-8 * np.abs(g_o).max()
+    assert np.abs(g - g_o).max() <= 1e-9 * np.abs(g_o).max()
     idx = np.random.default_rng(33).choice(np.flatnonzero(elem_vec(free)), 12, replace=False)
     for i in idx:
         up, lo = xv.copy(), xv.copy()
@@ -248,3 +248,76 @@ def test_autograd_gradient_with_protected_regions_matches_oracle_and_fd():
         fd = (cu - cl) / 2e-6
         assert g[i] == pytest.approx(fd, rel=1e-4, abs=1e-6 * np.abs(g).max())
     pr.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solver_non_convergence_raises_and_frees_the_plan(precision):
+    """A Krylov cap far below what the solve needs raises the reference's
+    RuntimeError("CG failed to converge ...") (pytopo3d/utils/solver.py:57-58),
+    and top3d releases the plan's device memory on that path too (the mixed
+    plan holds its ~2 GiB direction ring by then)."""
+    nelx, nely, nelz = 96, 48, 48
+    note("non-convergence", precision)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0, _ = torch.cuda.mem_get_info()
+    with pytest.raises(RuntimeError, match="CG failed to converge"):
+        top3d(nelx, nely, nelz, 0.3, 3.0, 2.0, 0.5, elem_size=1.0, maxloop=2,
+              beta_schedule=(1.0,), precision=precision, solver_rtol=1e-12, solver_maxit=2)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free1, _ = torch.cuda.mem_get_info()
+    note("  device memory not returned (MB)", (free0 - free1) / 1e6)
+    assert free0 - free1 < 64e6
+
+
+def test_solve_reuse_rejects_other_design_buffer_and_new_masks():
+    """TK_SOLVE_REUSE is accepted only for the set-up's own rho buffer, penal
+    and precision with the masks unchanged since."""
+    nelx, nely, nelz = 10, 6, 4
+    note("reuse checks")
+    F = edge_load_force_vector(nelx, nely, nelz)
+    z = np.zeros((nely, nelx, nelz), bool)
+    pr = _problem(nelx, nely, nelz, 1.5, z, z, F)
+    rng = np.random.default_rng(8)
+    rho = to_dev(rng.uniform(0.1, 1.0, pr.plan.nele))
+    other = rho.clone()
+    U = torch.zeros_like(pr.F)
+    pr.plan.prepare(rho, 3.0)
+    pr.plan.solve(rho, 3.0, pr.F, U, reuse=True)
+    with pytest.raises(ValueError, match="TK_SOLVE_REUSE"):
+        pr.plan.solve(other, 3.0, pr.F, U, reuse=True)
+    pr.plan.prepare(rho, 3.0)
+    from pytopo3d_amd.problem import element_flags
+    pr.plan.set_masks(to_dev(element_flags(z, z), torch.uint8),
+                      to_dev(support_node_flags(nelx, nely, nelz, None), torch.uint8))
+    with pytest.raises(ValueError, match="TK_SOLVE_REUSE"):
+        pr.plan.solve(rho, 3.0, pr.F, U, reuse=True)
+    with pytest.raises(ValueError, match="precision"):
+        top3d(nelx, nely, nelz, 0.3, 3.0, 1.5, 0.5, precision="fp32")
+    pr.close()
+
+
+def test_history_snapshots_are_the_per_iteration_densities():
+    """save_history copies the snapshot densities to the host asynchronously
+    (pinned buffers on a side stream, waited for only before their device
+    buffer is reused): a sparse snapshot schedule gives exactly the matching
+    entries of the every-iteration history, and those equal the densities the
+    iterations solved with as reported to the per-iteration callback."""
+    nelx, nely, nelz = 20, 10, 6
+    note("history snapshots")
+    F = edge_load_force_vector(nelx, nely, nelz)
+    kw = dict(elem_size=1.0, tolx=1e-12, maxloop=7, beta_schedule=(1.0, 4.0), force_vector=F,
+              save_history=True)
+    seen = []
+    _, h1, c1, _ = top3d(nelx, nely, nelz, 0.3, 3.0, 2.0, 0.5, history_frequency=1,
+                         iteration_callback=lambda d: seen.append(d["rho"].cpu().numpy()), **kw)
+    _, h4, c4, _ = top3d(nelx, nely, nelz, 0.3, 3.0, 2.0, 0.5, history_frequency=4, **kw)
+    assert h1["iteration_history"] == list(range(14)) + [14]
+    assert h4["iteration_history"] == [0, 4, 6, 8, 12, 13, 14]
+    for it, d in zip(h4["iteration_history"], h4["density_history"]):
+        assert d.shape == (nely, nelx, nelz) and d.dtype == np.float64
+        np.testing.assert_array_equal(d, h1["density_history"][it])
+    for it in range(14):
+        np.testing.assert_array_equal(elem_vec(h1["density_history"][it]), seen[it])
+    assert c1 == c4
