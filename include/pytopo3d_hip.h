@@ -106,6 +106,29 @@ int tk_plan_destroy(tk_plan* plan);
 int tk_plan_set_masks(tk_plan* plan, const uint8_t* elem_flags, const uint8_t* node_fixed,
                       long long* nfree, void* stream);
 
+/* Boundary conditions on the device from the reference's inputs, each of which
+ * may be NULL (the reference's None):
+ *   force_field          (nely, nelx, nelz, 3) doubles, C order (the reference's
+ *                        array as is); NULL: the default line load
+ *   support_mask         (nely, nelx, nelz) bytes, C order; NULL or all zero:
+ *                        the x = 0 face is fixed
+ *   obstacle_mask,
+ *   protected_zone_mask  (nely, nelx, nelz) bytes, C order
+ * Outputs (device, each may be NULL, at least one given):
+ *   F          ndof doubles: build_force_vector (pytopo3d/utils/assembly.py:78-184),
+ *              bitwise (each node's element contributions added in the
+ *              reference's loop order)
+ *   node_fixed nnode bytes as tk_plan_set_masks takes them: build_supports
+ *              (pytopo3d/utils/assembly.py:224-348)
+ *   elem_flags nele bytes in element order as tk_plan_set_masks takes them: the
+ *              protected-solid / void masks of pytopo3d/core/optimizer.py:343-405
+ *              (load and support elements or the default strips are solid,
+ *              void wins). */
+int tk_build_boundary_conditions(tk_plan* plan, const double* force_field,
+                                 const uint8_t* support_mask, const uint8_t* obstacle_mask,
+                                 const uint8_t* protected_zone_mask, double* F,
+                                 uint8_t* node_fixed, uint8_t* elem_flags, void* stream);
+
 /* Plan geometry: writes nlev and up to cap*3 element dims (nelx,nely,nelz per level). */
 int tk_plan_levels(const tk_plan* plan, int* nlev, int* dims, int cap);
 /* Stencil table (HOST outputs): dy,dx,dz per entry and the weight; returns count in *n. */

@@ -60,6 +60,7 @@ _SIGS = {
                             ctypes.POINTER(TkOptions)]),
     "tk_plan_destroy": (_I, [_P]),
     "tk_plan_set_masks": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_longlong), _P]),
+    "tk_build_boundary_conditions": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "tk_plan_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
     "tk_plan_stencil": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D), _I, ctypes.POINTER(_I)]),
     "tk_plan_filter_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -177,6 +178,14 @@ class Plan:
                                        ctypes.byref(n), _stream()))
         self.nfree = int(n.value)
         return self.nfree
+
+    def boundary_conditions(self, force_field=None, support_mask=None, obstacle_mask=None,
+                            protected_zone_mask=None, F=None, node_fixed=None, elem_flags=None):
+        """tk_build_boundary_conditions: inputs are device tensors in the
+        reference's layouts (C-order (nely, nelx, nelz[, 3])), any may be None."""
+        _check(lib().tk_build_boundary_conditions(
+            self._h, _ptr(force_field), _ptr(support_mask), _ptr(obstacle_mask),
+            _ptr(protected_zone_mask), _ptr(F), _ptr(node_fixed), _ptr(elem_flags), _stream()))
 
     def levels(self):
         n = _I()

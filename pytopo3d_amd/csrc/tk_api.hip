@@ -23,6 +23,9 @@ int smooth_step_fine(Plan& P, int prec, const void* b, const void* x, void* xo, 
 void launch_oc_init(Plan& P, double target, int max_steps, double tol, int probes, hipStream_t s);
 void launch_chain_weights(Plan& P, const double* gc, const double* gv, const double* drho,
                           double* wc, double* wv, hipStream_t s);
+void launch_boundary_conditions(Plan& P, const double* ff, const uint8_t* sm, const uint8_t* ob,
+                                const uint8_t* pz, double* F, uint8_t* fixed, uint8_t* flags,
+                                hipStream_t s);
 
 // Largest eigenvalue of diag(KE)^-1/2 KE diag(KE)^-1/2 by cyclic Jacobi rotations.
 static double lambda_bound(const double* KE) {
@@ -441,6 +444,21 @@ int tk_plan_set_masks(tk_plan* plan, const uint8_t* elem_flags, const uint8_t* n
   P.nfree = (long long)P.sc_host->sum0;
   P.masks_set = true;
   if (nfree) *nfree = P.nfree;
+  return 0;
+}
+
+int tk_build_boundary_conditions(tk_plan* plan, const double* force_field,
+                                 const uint8_t* support_mask, const uint8_t* obstacle_mask,
+                                 const uint8_t* protected_zone_mask, double* F,
+                                 uint8_t* node_fixed, uint8_t* elem_flags, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!F && !node_fixed && !elem_flags) {
+    set_error("at least one of F, node_fixed and elem_flags is required");
+    return 1;
+  }
+  tk::launch_boundary_conditions(P, force_field, support_mask, obstacle_mask, protected_zone_mask,
+                                 F, node_fixed, elem_flags, (hipStream_t)stream);
+  TK_LAUNCH_CHECK();
   return 0;
 }
 

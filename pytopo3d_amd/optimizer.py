@@ -36,7 +36,6 @@ import torch
 
 from . import _native
 from .problem import (
-    build_force_vector,
     element_flags,
     element_stiffness,
     protected_masks,
@@ -70,8 +69,14 @@ class SIMPProblem:
     """
 
     def __init__(self, nelx: int, nely: int, nelz: int, rmin: float, KE: np.ndarray,
-                 F: np.ndarray, node_fixed: np.ndarray, solid: np.ndarray, void: np.ndarray,
-                 solver_options: Optional[dict] = None):
+                 F: Optional[np.ndarray], node_fixed: Optional[np.ndarray], solid: np.ndarray,
+                 void: np.ndarray, solver_options: Optional[dict] = None,
+                 reference_bcs: Optional[dict] = None):
+        """``F`` and ``node_fixed`` are host arrays, or None with
+        ``reference_bcs`` = the reference's inputs (force_field, support_mask,
+        and for ``device_flags`` obstacle_mask and protected_zone_mask), from
+        which the loads, supports and, with ``device_flags``, the element
+        flags are built on the device (tk_build_boundary_conditions)."""
         self.dev = _device()
         self.nelx, self.nely, self.nelz = int(nelx), int(nely), int(nelz)
         self.shape = (self.nely, self.nelx, self.nelz)
@@ -80,14 +85,39 @@ class SIMPProblem:
         self.solid = np.asarray(solid, dtype=bool)
         self.void = np.asarray(void, dtype=bool)
         self.free = ~(self.solid | self.void)
-        flags = torch.from_numpy(element_flags(self.solid, self.void)).to(self.dev)
-        fixed = torch.from_numpy(np.ascontiguousarray(node_fixed, dtype=np.uint8)).to(self.dev)
+        if reference_bcs is None:
+            flags = torch.from_numpy(element_flags(self.solid, self.void)).to(self.dev)
+            fixed = torch.from_numpy(np.ascontiguousarray(node_fixed, dtype=np.uint8)).to(self.dev)
+            self.F = torch.from_numpy(np.ascontiguousarray(F, dtype=np.float64)).to(self.dev)
+        else:
+            self.F, fixed, flags = self._device_bcs(**reference_bcs)
         self.nfree = self.plan.set_masks(flags, fixed)
-        self.F = torch.from_numpy(np.ascontiguousarray(F, dtype=np.float64)).to(self.dev)
+        if self.nfree != int(np.sum(self.free)):
+            raise RuntimeError(f"device element flags give {self.nfree} free elements, the host "
+                               f"masks {int(np.sum(self.free))}")
         self.U = torch.zeros(self.plan.ndof, dtype=torch.float64, device=self.dev)
         self.have_U = False
         self.last_iters = 0
         self.last_relres = 0.0
+
+    def _device_bcs(self, force_field=None, support_mask=None, obstacle_mask=None,
+                    protected_zone_mask=None, device_flags=False):
+        def up(a, dtype):
+            if a is None:
+                return None
+            a = np.asarray(a, dtype=bool) if dtype is np.uint8 else np.asarray(a)
+            return torch.from_numpy(np.ascontiguousarray(a.astype(dtype))).to(self.dev)
+
+        F = torch.empty(self.plan.ndof, dtype=torch.float64, device=self.dev)
+        fixed = torch.empty(self.plan.nnode, dtype=torch.uint8, device=self.dev)
+        flags = (torch.empty(self.plan.nele, dtype=torch.uint8, device=self.dev) if device_flags
+                 else torch.from_numpy(element_flags(self.solid, self.void)).to(self.dev))
+        self.plan.boundary_conditions(
+            force_field=up(force_field, np.float64), support_mask=up(support_mask, np.uint8),
+            obstacle_mask=up(obstacle_mask, np.uint8) if device_flags else None,
+            protected_zone_mask=up(protected_zone_mask, np.uint8) if device_flags else None,
+            F=F, node_fixed=fixed, elem_flags=flags if device_flags else None)
+        return F, fixed, flags
 
     def new_elem(self) -> torch.Tensor:
         return torch.empty(self.plan.nele, dtype=torch.float64, device=self.dev)
@@ -203,10 +233,25 @@ def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float
 
 
 def _make_problem(nelx, nely, nelz, rmin, material_params, elem_size, force_field, support_mask,
-                  solid, void, force_vector, solver_options) -> SIMPProblem:
+                  solid, void, force_vector, solver_options, obstacle_mask=None,
+                  protected_zone_mask=None, device_flags=False) -> SIMPProblem:
+    """The device problem.  Loads and supports from the reference's inputs are
+    built on the device (and, with ``device_flags``, the protected-element
+    flags of top3d, which ``solid`` / ``void`` mirror on the host for the
+    bookkeeping); an explicit ``force_vector`` is uploaded as is."""
     KE = element_stiffness(material_params, elem_size)
-    F = (build_force_vector(nelx, nely, nelz, force_field) if force_vector is None
-         else np.asarray(force_vector, dtype=float).ravel())
+    shape = (nely, nelx, nelz)
+    if force_field is not None and np.shape(force_field) != shape + (3,):
+        raise ValueError(f"force_field has shape {np.shape(force_field)}, but expected {shape + (3,)}")
+    if support_mask is not None and np.shape(support_mask) != shape:
+        raise ValueError(f"support_mask has shape {np.shape(support_mask)}, but expected {shape}")
+    if force_vector is None:
+        bcs = dict(force_field=force_field, support_mask=support_mask,
+                   obstacle_mask=obstacle_mask, protected_zone_mask=protected_zone_mask,
+                   device_flags=device_flags)
+        return SIMPProblem(nelx, nely, nelz, rmin, KE, None, None, solid, void, solver_options,
+                           reference_bcs=bcs)
+    F = np.asarray(force_vector, dtype=float).ravel()
     if F.size != 3 * (nelx + 1) * (nely + 1) * (nelz + 1):
         raise ValueError(f"force_vector has {F.size} entries, expected {3 * (nelx + 1) * (nely + 1) * (nelz + 1)}")
     fixed = support_node_flags(nelx, nely, nelz, support_mask)
@@ -372,7 +417,9 @@ def top3d(
 
     t_setup = time.perf_counter()
     problem = _make_problem(nelx, nely, nelz, rmin, material_params, elem_size, force_field,
-                            support_mask, solid, void, force_vector, solver_options)
+                            support_mask, solid, void, force_vector, solver_options,
+                            obstacle_mask=obstacle_mask, protected_zone_mask=protected_zone_mask,
+                            device_flags=True)
     try:
         plan = problem.plan
         dev = problem.dev

@@ -213,3 +213,40 @@ def element_flags(solid: np.ndarray, void: np.ndarray) -> np.ndarray:
     f[solid] |= SOLID
     f[void] |= VOID
     return f.ravel(order="F")
+
+
+def _check_box(nelx, nely, nelz, x1, x2, y1, y2, z1, z2):
+    if not (0 <= x1 < x2 <= nelx and 0 <= y1 < y2 <= nely and 0 <= z1 < z2 <= nelz):
+        raise ValueError("Specified index range is out of bounds")
+
+
+def build_force_field(nelx: int, nely: int, nelz: int, x1: int, x2: int, y1: int, y2: int,
+                      z1: int, z2: int, F_x: float, F_y: float, F_z: float) -> np.ndarray:
+    """(nely, nelx, nelz, 3) element force field of a preset: the force vector
+    divided evenly over the elements of the box [y1:y2, x1:x2, z1:z2]
+    (pytopo3d/utils/assembly.py:28-74)."""
+    _check_box(nelx, nely, nelz, x1, x2, y1, y2, z1, z2)
+    ff = np.zeros((nely, nelx, nelz, 3), dtype=float)
+    n = (x2 - x1) * (y2 - y1) * (z2 - z1)
+    ff[y1:y2, x1:x2, z1:z2] = (F_x / n, F_y / n, F_z / n)
+    return ff
+
+
+def build_support_mask(nelx: int, nely: int, nelz: int, x1: int, x2: int, y1: int, y2: int,
+                       z1: int, z2: int) -> np.ndarray:
+    """(nely, nelx, nelz) support mask of a preset box
+    (pytopo3d/utils/assembly.py:186-222)."""
+    _check_box(nelx, nely, nelz, x1, x2, y1, y2, z1, z2)
+    m = np.zeros((nely, nelx, nelz), dtype=bool)
+    m[y1:y2, x1:x2, z1:z2] = True
+    return m
+
+
+def protected_zone_mask_from_ranges(nelx: int, nely: int, nelz: int,
+                                    zones: Sequence[Sequence[int]]) -> np.ndarray:
+    """Union of protected-zone boxes (x1, x2, y1, y2, z1, z2), each set as
+    mask[y1:y2, x1:x2, z1:z2] (the custom-mask path of main.py:158-168)."""
+    m = np.zeros((nely, nelx, nelz), dtype=bool)
+    for x1, x2, y1, y2, z1, z2 in zones:
+        m[y1:y2, x1:x2, z1:z2] = True
+    return m

@@ -321,3 +321,30 @@ def test_history_snapshots_are_the_per_iteration_densities():
     for it in range(14):
         np.testing.assert_array_equal(elem_vec(h1["density_history"][it]), seen[it])
     assert c1 == c4
+
+
+def test_runner_execute_optimization_reproduces_reference_and_saves_result(tmp_path):
+    """execute_optimization with the reference runner's keyword set
+    (pytopo3d/runners/experiment.py:95-213, forwarded to top3d at :173-195)
+    reproduces the reference's golden trajectory, returns its 5-tuple, and
+    save_result writes the (nely, nelx, nelz) float64 design as .npy
+    (pytopo3d/utils/results_manager.py:86-104)."""
+    from pytopo3d_amd.runner import execute_optimization, save_result
+    note("runner")
+    name, kw, z, meta = [g for g in GOLD if g[0] == "default_bc_6x4x3"][0]
+    args = dict(kw)
+    args["create_animation"] = args.pop("save_history")
+    args["animation_frequency"] = args.pop("history_frequency")
+    diag = {}
+    xPhys, hist, c, failure, run_time = execute_optimization(
+        logger=None, combined_obstacle_mask=None, use_gpu=True, protected_zone_mask=None,
+        diagnostics_out=diag, solver_rtol=1e-14, **args)
+    assert failure is None and run_time > 0.0
+    np.testing.assert_allclose(xPhys, z["rho"], atol=1e-8)
+    np.testing.assert_allclose(hist["compliance_history"], z["compliance_history"], rtol=1e-9)
+    assert c == pytest.approx(meta["final_compliance"], rel=1e-9)
+    path = save_result(xPhys, str(tmp_path / "exp"))
+    back = np.load(path)
+    assert path.endswith("optimized_design.npy")
+    assert back.dtype == np.float64 and back.shape == z["rho"].shape
+    np.testing.assert_array_equal(back, xPhys)

@@ -140,3 +140,42 @@ def test_oc_bisection_with_probes_equals_plain_bisection(tmp_path):
     for ln in smooth:
         plain, probes = map(float, re.findall(r"plain ([\d.]+) probes ([\d.]+)", ln)[0])
         assert probes < 0.3 * plain, ln
+
+
+def test_runner_signature_matches_the_reference_runner():
+    """execute_optimization takes the reference runner's parameters in its
+    order with its defaults (pytopo3d/runners/experiment.py:95-119)."""
+    import inspect
+
+    from pytopo3d_amd.runner import execute_optimization
+    want = [("nelx", None), ("nely", None), ("nelz", None), ("volfrac", None), ("penal", None),
+            ("rmin", None), ("disp_thres", None), ("material_params", None),
+            ("elem_size", 0.01), ("force_field", None), ("support_mask", None), ("tolx", 0.01),
+            ("maxloop", 2000), ("create_animation", False), ("animation_frequency", 10),
+            ("logger", None), ("combined_obstacle_mask", None), ("use_gpu", False),
+            ("protected_zone_mask", None), ("beta_schedule", (1.0, 2.0, 4.0, 8.0)),
+            ("projection_eta", 0.5), ("move", 0.2), ("diagnostics_out", None)]
+    params = [p for p in inspect.signature(execute_optimization).parameters.values()
+              if p.kind is not inspect.Parameter.VAR_KEYWORD]
+    assert [p.name for p in params] == [n for n, _ in want]
+    for p, (_, d) in zip(params, want):
+        if p.default is not inspect.Parameter.empty:
+            assert p.default == d, p.name
+
+
+def test_preset_helpers_match_reference_semantics():
+    """Preset boxes of pytopo3d/utils/assembly.py:28-74, :186-222 and the
+    protected-zone ranges of main.py:158-168."""
+    from pytopo3d_amd.problem import (build_force_field, build_support_mask,
+                                      protected_zone_mask_from_ranges)
+    n = (12, 6, 4)
+    ff = build_force_field(*n, 10, 12, 0, 6, 0, 4, 1.0, -2.0, 0.0)
+    assert ff.shape == (6, 12, 4, 3)
+    np.testing.assert_allclose(ff.reshape(-1, 3).sum(axis=0), [1.0, -2.0, 0.0])
+    assert np.count_nonzero(ff[..., 0]) == 2 * 6 * 4
+    with pytest.raises(ValueError, match="out of bounds"):
+        build_force_field(*n, 10, 13, 0, 6, 0, 4, 1.0, 0.0, 0.0)
+    with pytest.raises(ValueError, match="out of bounds"):
+        build_support_mask(*n, 3, 3, 0, 6, 0, 4)
+    m = protected_zone_mask_from_ranges(*n, [(0, 2, 0, 6, 0, 4), (1, 3, 2, 3, 1, 2)])
+    assert m.sum() == 2 * 6 * 4 + 1
