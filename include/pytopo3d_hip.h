@@ -205,6 +205,24 @@ int tk_energy(tk_plan* plan, const double* U, const double* rho, const double* d
               double* ce, double* dc_drho, double* dc_dx, double* dv_dx, double* c_out,
               void* stream);
 
+/* Allocation-free variant of tk_energy without a plan: ce, dc_drho (device
+ * outputs, each may be NULL) and c = sum_e s_e ce of a displacement field U on
+ * a nelx x nely x nelz grid.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  KE: HOST pointer, 576
+ * doubles, with the element's mirror symmetry (every axis-aligned lk_H8 KE).
+ * c_dev (device, 1 double) and/or c_host (host; synchronises the stream, so
+ * not under capture) receive c; either may be NULL.
+ * Limit: nelx*nely*nelz <= 16,777,216 (the library's static scratch: 64 Ki
+ * partial sums plus 1 KiB of scalars, 513 KiB held on each device it runs on
+ * until the process ends); larger grids return 1.  It calls no allocating
+ * routine.  Every call on a device shares that scratch, which keeps the last
+ * call's contents: calls that overlap on one device (several streams or host
+ * threads) race and must be serialised by the caller. */
+int tk_energy_noalloc(int nelx, int nely, int nelz, const double* KE, const double* U,
+                      const double* rho, double penal, double* ce, double* dc_drho,
+                      double* c_dev, double* c_host, void* stream);
+
 /* Projected-volume OC update (optimality_criteria_update_projected,
  * pytopo3d/utils/oc_update.py:25-124), the bisection entirely on the device.
  * x, dc_dx, dv_dx: nele doubles; x_new: output.  target: free-volume target

@@ -79,6 +79,8 @@ _SIGS = {
     "tk_volume_bounds": (_I, [_P, _D, _D, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "tk_read_results": (_I, [_P, ctypes.POINTER(_D), _I, _P]),
     "tk_plan_counters": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I]),
+    "tk_energy_noalloc": (_I, [_I, _I, _I, ctypes.POINTER(_D), _P, _P, _D, _P, _P, _P,
+                               ctypes.POINTER(_D), _P]),
     "tk_last_error": (ctypes.c_char_p, []),
 }
 EXPORTED = tuple(_SIGS)
@@ -289,3 +291,18 @@ class Plan:
         keys = ("applies64", "applies32", "cg_iters", "ir_steps", "solves", "oc_steps",
                 "filter_passes")
         return dict(zip(keys, list(out)))
+
+def energy_noalloc(nelx: int, nely: int, nelz: int, KE: np.ndarray, U: torch.Tensor,
+                   rho: torch.Tensor, penal: float, ce: Optional[torch.Tensor] = None,
+                   dc_drho: Optional[torch.Tensor] = None, c_dev: Optional[torch.Tensor] = None,
+                   sync: bool = True) -> Optional[float]:
+    """tk_energy_noalloc: energies and compliance without a plan or any device
+    allocation (the library's static scratch holds the reduction).  Returns c
+    when ``sync`` (the stream is synchronised), else None (c goes to c_dev)."""
+    ke = np.ascontiguousarray(np.asarray(KE, dtype=np.float64).reshape(24, 24))
+    c = _D()
+    _check(lib().tk_energy_noalloc(int(nelx), int(nely), int(nelz),
+                                   ke.ctypes.data_as(ctypes.POINTER(_D)), _ptr(U), _ptr(rho),
+                                   float(penal), _ptr(ce), _ptr(dc_drho), _ptr(c_dev),
+                                   ctypes.byref(c) if sync else None, _stream()))
+    return float(c.value) if sync else None

@@ -115,6 +115,18 @@ __global__ __launch_bounds__(kBlock) void k_energy_sector(
   block_partials<1>(part, partials);
 }
 
+// k_energy_sector without a plan (c's partials and the finalize scalars in
+// caller-provided device memory).
+void launch_energy_sector_raw(const Grid& g, const double* sec, const double* U,
+                              const double* rho, double penal, double* ce, double* dc_drho,
+                              double* partials, Scalars* sc, hipStream_t s) {
+  SparseKE<double> kb;
+  for (int k = 0; k < kSecN; ++k) kb.v[k] = sec[k];
+  hipLaunchKernelGGL(k_energy_sector, dim3(nblocks(g.nele)), dim3(kBlock), 0, s, g, kb, U, rho,
+                     nullptr, nullptr, nullptr, penal, -penal * (1.0 - kEminE), ce, dc_drho,
+                     nullptr, nullptr, partials, sc);
+}
+
 void launch_energy(Plan& P, const double* U, const double* rho, const double* drho, double penal,
                    double* ce, double* dc_drho, double* wc, double* wv, hipStream_t s) {
   const double c0 = -penal * (1.0 - kEminE);

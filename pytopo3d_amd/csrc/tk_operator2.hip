@@ -509,7 +509,7 @@ static bool build_sectors(const double* KE, double out[kSecN]) {
     return v;
   };
   // B[(m,c),(m',c')] = sum_{a,b} psi(a,m) KE[(a,c),(b,c')] psi(b,m') / 64
-  static double B[24][24];
+  double B[24][24];
   double bmax = 0.0;
   for (int m = 0; m < 8; ++m)
     for (int c = 0; c < 3; ++c)
@@ -543,6 +543,10 @@ static bool build_sectors(const double* KE, double out[kSecN]) {
       if (!kept[i][j]) off = fmax(off, fabs(B[i][j]));
   return off <= 1e-12 * bmax;
 }
+
+// The 45 sector entries of KE (host), false when KE lacks the element's
+// mirror symmetry.
+bool sector_form(const double* KE, double out[kSecN]) { return build_sectors(KE, out); }
 
 bool operator_setup(Plan& P) {
   double sec[kSecN];

@@ -138,6 +138,11 @@ __global__ __launch_bounds__(kFinThreadsMax) void k_finalize(int op, int G,
   }
 }
 
+void finalize_raw(int op, int G, const double* partials, Scalars* sc, hipStream_t s) {
+  const int threads = G > 256 * kFinLoads ? kFinThreadsMax : 256;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(threads), 0, s, op, G, partials, sc, 0.0, 0);
+}
+
 void finalize(Plan& P, int op, int G, hipStream_t s, double a0, int i0, const double* partials) {
   const int threads = G > 256 * kFinLoads ? kFinThreadsMax : 256;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(threads), 0, s, op, G,
