@@ -590,8 +590,10 @@ class MultigridCG:
 
 def fixed_geometry_metrics(xPhys, penal, material_params=None, elem_size=0.01,
                            force_field=None, support_mask=None, obstacle_mask=None,
-                           protected_zone_mask=None, force_vector_override=None) -> Dict[str, Any]:
-    """One FE solve on a fixed field (pytopo3d/core/optimizer.py:54-252)."""
+                           protected_zone_mask=None, force_vector_override=None,
+                           solver: Optional[Callable] = None) -> Dict[str, Any]:
+    """One FE solve on a fixed field (pytopo3d/core/optimizer.py:54-252);
+    ``solver`` as for solve_displacement (default SuperLU)."""
     nely, nelx, nelz = xPhys.shape
     shape = (nely, nelx, nelz)
     obstacle_mask = np.zeros(shape, bool) if obstacle_mask is None else obstacle_mask
@@ -607,7 +609,7 @@ def fixed_geometry_metrics(xPhys, penal, material_params=None, elem_size=0.01,
     F_total = float(np.sum(np.abs(F[loaded]))) if loaded.size else 0.0
     free, _ = support_dofs(nelx, nely, nelz, support_mask)
     KE = element_stiffness(material_params, elem_size)
-    U = solve_displacement(xe, penal, KE, F, free, nelx, nely, nelz)
+    U = solve_displacement(xe, penal, KE, F, free, nelx, nely, nelz, solver)
     ce = element_energy(U, element_dofs(nelx, nely, nelz), KE).reshape(shape, order="F")
     c = float(((EMIN + xe ** penal * (1.0 - EMIN)) * ce).sum())
     uavg = [float(np.mean(U[3 * nodes + a])) if nodes.size else None for a in range(3)]
@@ -801,7 +803,7 @@ def top3d(nelx, nely, nelz, volfrac, penal, rmin, disp_thres, material_params=No
         return rho, history, float("nan"), None
     kw = dict(material_params=material_params, elem_size=elem_size, force_field=force_field,
               support_mask=support_mask, obstacle_mask=void, protected_zone_mask=solid,
-              force_vector_override=force_vector_override)
+              force_vector_override=force_vector_override, solver=solver)
     final_c = fixed_geometry_compliance(rho, penal, **kw)
     if save_history:
         history["density_history"].append(rho.copy())

@@ -14,9 +14,8 @@ against SuperLU on smaller grids in tests/test_oracle_solve_top3d.py).
 Stored (data only): per design iteration the compliance, the free volume
 fraction of the projected density the iteration solved with, the change and
 three sums of the design x after the iteration (plain, of squares, and
-weighted by default_rng(60).uniform(-1, 1, nele)); the full x and the
-projected density after iterations 1, 25, 50 and 100; the final projected
-density and compliance.
+weighted by default_rng(60).uniform(-1, 1, nele)); the full x after
+iterations 1, 25, 50 and 100; the final projected density and compliance.
 """
 
 from __future__ import annotations
@@ -44,7 +43,7 @@ def main():
     nele = NELX * NELY * NELZ
     w = np.random.default_rng(60).uniform(-1.0, 1.0, nele)
     rows = {k: [] for k in ("c", "vol", "change", "beta", "x_sum", "x_sq", "x_w")}
-    keep_x, keep_rho = {}, {}
+    keep_x = {}
     t0 = time.time()
     _, _, free_m = oracle.boundary_protection(NELX, NELY, NELZ, None, None, None, None, F)
 
@@ -59,7 +58,6 @@ def main():
         rows["x_w"].append(float(np.dot(w, x)))
         if d["total"] in KEEP:
             keep_x[d["total"]] = x.copy()
-            keep_rho[d["total"]] = d["rho_new"].ravel(order="F").copy()
         print(f"iter {d['total']:3d} beta {d['beta']:g} c {d['c']:.12e} "
               f"cg {solver.iterations[-1]} t {time.time() - t0:.0f}s", flush=True)
 
@@ -70,7 +68,6 @@ def main():
     out = {k: np.asarray(v) for k, v in rows.items()}
     out["keep_iters"] = np.asarray(KEEP)
     out["keep_x"] = np.stack([keep_x[k] for k in KEEP])
-    out["keep_rho"] = np.stack([keep_rho[k] for k in KEEP])
     out["final_rho"] = np.asarray(rho).ravel(order="F")
     out["final_c"] = np.asarray(c)
     out["cg_iterations"] = np.asarray(solver.iterations)

@@ -17,6 +17,7 @@ import pytest
 import torch
 
 import oracle
+from golden_cases import METRIC_CASES, check_metrics
 from gpu_helpers import dev, elem_vec, note, to_dev
 from pytopo3d_amd import (
     SIMPProblem,
@@ -348,3 +349,12 @@ def test_runner_execute_optimization_reproduces_reference_and_saves_result(tmp_p
     assert path.endswith("optimized_design.npy")
     assert back.dtype == np.float64 and back.shape == z["rho"].shape
     np.testing.assert_array_equal(back, xPhys)
+
+
+@pytest.mark.parametrize("name,x,kw,want", METRIC_CASES, ids=[c[0] for c in METRIC_CASES])
+def test_fixed_geometry_metrics_match_reference_fixtures(name, x, kw, want):
+    """evaluate_fixed_geometry_metrics against the dicts the reference's own
+    function returned on the same designs and inputs
+    (tests/golden/make_metrics_golden.py, pytopo3d/core/optimizer.py:54-252)."""
+    note("metrics golden", name)
+    check_metrics(evaluate_fixed_geometry_metrics(x, **kw), want)

@@ -15,6 +15,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 import oracle
+from golden_cases import METRIC_CASES, check_metrics
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -230,3 +231,10 @@ def test_multigrid_cg_matches_direct_solve(levels_dofs):
     u2 = mg(Kff, F[free])        # warm start from the previous solution
     assert np.abs(u2 - ud).max() <= 1e-9 * np.abs(ud).max()
     assert mg.iterations[1] <= 2
+
+
+@pytest.mark.parametrize("name,x,kw,want", METRIC_CASES, ids=[c[0] for c in METRIC_CASES])
+def test_oracle_fixed_geometry_metrics_match_reference(name, x, kw, want):
+    """The oracle's fixed_geometry_metrics against the dicts the reference's
+    evaluate_fixed_geometry_metrics returned (tests/golden/make_metrics_golden.py)."""
+    check_metrics(oracle.fixed_geometry_metrics(x, **kw), want)
