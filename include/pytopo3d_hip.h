@@ -63,7 +63,11 @@ typedef struct tk_options {
   int coarse_max_dof;     /* coarsest grid DOF limit, <= 192 (192) */
   int check_every;        /* iterations between host convergence checks (8) */
   int operator_kind;      /* 0: element kernel in the Walsh-Hadamard sector form when KE is
-                             mirror-symmetric (default), 1: dense node-gather kernel */
+                             mirror-symmetric (default), 1: dense node-gather kernel,
+                             2: as 0, with the FP32 sweeps (the mixed solve's multigrid,
+                             tk_apply_K and tk_smoother_step in TK_MIXED_FP32) contracting
+                             each element as a BF16 [elements x 32] x [32 x 32] MFMA
+                             product with FP32 accumulation (FP64 sweeps unchanged) */
   int no_graphs;          /* 0: each chunk of check_every Krylov iterations is captured once
                              per plan as a HIP graph and replayed (default); 1: kernels are
                              launched one by one */

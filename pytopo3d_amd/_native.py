@@ -292,6 +292,7 @@ class Plan:
                 "filter_passes")
         return dict(zip(keys, list(out)))
 
+
 def energy_noalloc(nelx: int, nely: int, nelz: int, KE: np.ndarray, U: torch.Tensor,
                    rho: torch.Tensor, penal: float, ce: Optional[torch.Tensor] = None,
                    dc_drho: Optional[torch.Tensor] = None, c_dev: Optional[torch.Tensor] = None,

@@ -74,7 +74,7 @@ static void free_plan(Plan* P) {
   if (P->cstream) (void)hipStreamSynchronize(P->cstream);   // a factorisation may be running
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(P->KE64); f(P->KE32); f(P->sten_off); f(P->sten_w); f(P->Hs); f(P->eflags); f(P->fcols);
-  f(P->sec64_dev); f(P->sec32_dev);
+  f(P->sec64_dev); f(P->sec32_dev); f(P->kemf);
   for (int l = 0; l < kMaxLevels; ++l) {
     Level& L = P->lv[l];
     f(L.fixed);
@@ -154,7 +154,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->no_graphs == 0 || o->no_graphs == 1) P->opt.no_graphs = o->no_graphs;
     if (o->filter_kind == 0 || o->filter_kind == 1) P->opt.filter_kind = o->filter_kind;
     if (o->check_every > 0) P->opt.check_every = o->check_every;
-    if (o->operator_kind == 0 || o->operator_kind == 1) P->opt.operator_kind = o->operator_kind;
+    if (o->operator_kind >= 0 && o->operator_kind <= 2) P->opt.operator_kind = o->operator_kind;
     if (o->direction_slots != 0 && (o->direction_slots < 2 || o->direction_slots > kRingMax)) {
       delete P;
       set_error("direction_slots must be 0 or in [2, " + std::to_string(kRingMax) + "]");
@@ -187,7 +187,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   P->lam_bound = lambda_bound(KE);
   operator_setup(*P);
   // the element kernel's smoothing forms D^-1 from KE's corner-independent diagonal
-  P->use_el = P->sector_ok && P->kd_ok && P->opt.operator_kind == 0;
+  P->use_el = P->sector_ok && P->kd_ok && P->opt.operator_kind != 1;
+  P->mfma = P->use_el && P->opt.operator_kind == 2 && P->kemf;
   P->rmin = rmin;
   // stencil: integer offsets with rmin - |d| > 1e-9, ordered (dz, dx, dy) ascending
   const int R = (int)std::ceil(rmin);

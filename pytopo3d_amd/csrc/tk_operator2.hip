@@ -32,6 +32,7 @@
 #include "tk_sector.hpp"
 
 #include <type_traits>
+#include <vector>
 
 namespace tk {
 
@@ -86,6 +87,28 @@ constexpr int kTile = 16;          // threads per tile edge
 constexpr int kOut = kTile - 1;    // owned nodes per tile edge
 constexpr int kPl = kTile + 1;     // node-plane edge held in LDS
 
+// BF16 MFMA form of the FP32 sweeps (operator_kind 2).  Per element layer a
+// wave evaluates its 64 elements as two 32-element blocks of the product
+// Y = KE' U on v_mfma_f32_32x32x16_bf16 (two K steps each), FP32 accumulation:
+//   rows/columns of KE' = 4 m + r, corner m = 4 oz + 2 oy + ox, component r
+//   (r = 3 is a zero pad), so a K step of 16 holds one face (4 corners) and
+//   lane half h = lane / 32 supplies the corners with oy = h of its element
+//   column (two nodes, from the LDS node plane), and receives back the outputs
+//   of the corners with ox = h (C rows 8 g + 4 h + r: corner 2 g + h).
+// KE' is staged per lane in the order the A operand wants (kemf: 64 lanes x
+// 2 steps x 8 bf16).  The outputs are scaled by the element scale in FP32 and
+// summed per node through LDS in a fixed order.
+typedef __bf16 mf_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float mf_f32x16 __attribute__((ext_vector_type(16)));
+typedef float mf_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf_bf16x8 mf_pack(const float a[3], const float b[3]) {
+  mf_bf16x8 v;
+  v[0] = (__bf16)a[0]; v[1] = (__bf16)a[1]; v[2] = (__bf16)a[2]; v[3] = (__bf16)0.0f;
+  v[4] = (__bf16)b[0]; v[5] = (__bf16)b[1]; v[6] = (__bf16)b[2]; v[7] = (__bf16)0.0f;
+  return v;
+}
+
 // Registers of one node plane.  `fetch` only issues loads into them; the
 // operand is formed and masked in `store`, a layer later, so no wave waits on
 // loads that were just issued (the wait counter is in order: one early wait
@@ -117,15 +140,18 @@ struct LayerIn {
   T eb[3], ed[3];
 };
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool MF = false>
 __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& Kv,
                                          const T* __restrict__ Kg, const T* __restrict__ s,
                                          const uint8_t* __restrict__ fixed,
                                          const T* __restrict__ u, const Epilogue<T>& ep, int nty,
                                          int zc, int with_dot, const int* __restrict__ done,
-                                         double* partials) {
+                                         double* partials, const uint4* __restrict__ kemf) {
+  static_assert(!MF || sizeof(T) == 4, "the MFMA form is an FP32 sweep");
   __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked operand)
-  __shared__ T yo[3][3][kTile * kTile];   // exchanged corner contributions (f = 1, 2, 3)
+  // exchanged corner contributions: sector form (f = 1, 2, 3), or the MFMA
+  // form's per-element corner outputs [oz][oy][ox][component]
+  __shared__ T yo[MF ? 24 : 9][kTile * kTile];
   __shared__ T kb[kSecN];                 // FP64 sector entries
   __shared__ T ss[kTile * kTile];         // element scales of the layer (on-the-fly diagonal)
   constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep || MODE == kModeChebFirstP;
@@ -336,11 +362,32 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   T F0[3][4], F1[3][4];   // face spectra of two consecutive node planes
   T Gp[3][4];             // top-face output spectrum of the previous element layer
   T s_prev = T(0);        // on-the-fly diagonal: scale sum of the previous layer
-  face_spectrum(0, F0);
+  if constexpr (!MF) face_spectrum(0, F0);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int f = 0; f < 4; ++f) Gp[c][f] = T(0);
+  // MFMA form: this lane supplies K half mh of element columns eA (block A)
+  // and eB (block B); p* are the LDS plane offsets of node (ly_e + mh, lx_e)
+  const int mlane = tid & 63, mh = mlane >> 5;
+  const int eA = (tid & ~63) + (mlane & 31), eB = eA + 32;
+  const int pA = (eA & (kTile - 1)) + mh + kPl * (eA >> 4);
+  const int pB = (eB & (kTile - 1)) + mh + kPl * (eB >> 4);
+  mf_bf16x8 ka0{}, ka1{}, bA{}, bB{};
+  T top_prev[3] = {T(0), T(0), T(0)};   // top-corner outputs of the previous layer at the own node
+  auto mf_face = [&](int b, int p) -> mf_bf16x8 {
+    const float a0[3] = {(float)un[b][0][p], (float)un[b][1][p], (float)un[b][2][p]};
+    const float a1[3] = {(float)un[b][0][p + kPl], (float)un[b][1][p + kPl],
+                         (float)un[b][2][p + kPl]};
+    return mf_pack(a0, a1);
+  };
+  if constexpr (MF) {
+    const uint4 q0 = kemf[2 * mlane], q1 = kemf[2 * mlane + 1];
+    __builtin_memcpy(&ka0, &q0, sizeof(q0));
+    __builtin_memcpy(&ka1, &q1, sizeof(q1));
+    bA = mf_face(0, pA);
+    bB = mf_face(0, pB);
+  }
 
   // Element layer ez.  Fb: spectrum of plane ez (bottom), Ft: filled with that
   // of plane ez + 1 (LDS buffer `buf`); nxt holds plane ez + 1 (registers),
@@ -372,38 +419,78 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
     // its plane, still in LDS, instead of being carried from the previous
     // layer (12 doubles fewer live across the sector blocks, which spilled);
     // the FP64 smoothing sweeps run at 2 waves without spilling and carry it
-    if constexpr (sizeof(T) == 8 && (MODE == kModeApply || MODE == kModeApplyDir))
-      face_spectrum(buf ^ 1, Fb);
-    face_spectrum(buf, Ft);
-    T a[3][8], o[3][8];
+    T own0[3] = {T(0), T(0), T(0)};   // sector form: the own corner's output (bottom + previous top)
+    if constexpr (MF) {
+      // the two blocks' products on the matrix cores, the top face read now
+      // and the bottom face carried from the previous layer
+      const mf_bf16x8 tA = mf_face(buf, pA), tB = mf_face(buf, pB);
+      mf_f32x16 yA{}, yB{};
+      yA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ka0, bA, yA, 0, 0, 0);
+      yB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ka0, bB, yB, 0, 0, 0);
+      yA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ka1, tA, yA, 0, 0, 0);
+      yB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ka1, tB, yB, 0, 0, 0);
+      bA = tA;
+      bB = tB;
+      // element scales of the two columns (the partner lane holds the other)
+      const T sp = __shfl_xor(se, 32, 64);
+      const T sA = mh ? sp : se, sB = mh ? se : sp;
+      // C row 8 g + 4 mh + r is component r of corner m = 2 g + mh
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+      for (int gq = 0; gq < 4; ++gq)
 #pragma unroll
-      for (int f = 1; f < 4; ++f) a[c][f] = Fb[c][f] + Ft[c][f];
+        for (int r = 0; r < 3; ++r) {
+          yo[(2 * gq + mh) * 3 + r][eA] = sA * yA[4 * gq + r];
+          yo[(2 * gq + mh) * 3 + r][eB] = sB * yB[4 * gq + r];
+        }
+    } else {
+      if constexpr (sizeof(T) == 8 && (MODE == kModeApply || MODE == kModeApplyDir))
+        face_spectrum(buf ^ 1, Fb);
+      face_spectrum(buf, Ft);
+      T a[3][8], o[3][8];
 #pragma unroll
-      for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
-    }
-    if constexpr (sizeof(T) == 8) sector_apply<T>(kb, se, a, o);
-    else sector_apply<T>(Kv.v, se, a, o);
-    // bottom plane: sum with the previous layer's top, inverse face transform
-    T nb[3][4];
+      for (int c = 0; c < 3; ++c) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      nb[c][0] = Gp[c][0] - o[c][4];
-      Gp[c][0] = o[c][4];
+        for (int f = 1; f < 4; ++f) a[c][f] = Fb[c][f] + Ft[c][f];
 #pragma unroll
-      for (int f = 1; f < 4; ++f) {
-        nb[c][f] = Gp[c][f] + (o[c][f] - o[c][f + 4]);
-        Gp[c][f] = o[c][f] + o[c][f + 4];
+        for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
       }
-      face_inv(nb[c]);
+      if constexpr (sizeof(T) == 8) sector_apply<T>(kb, se, a, o);
+      else sector_apply<T>(Kv.v, se, a, o);
+      // bottom plane: sum with the previous layer's top, inverse face transform
+      T nb[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nb[c][0] = Gp[c][0] - o[c][4];
+        Gp[c][0] = o[c][4];
+#pragma unroll
+        for (int f = 1; f < 4; ++f) {
+          nb[c][f] = Gp[c][f] + (o[c][f] - o[c][f + 4]);
+          Gp[c][f] = o[c][f] + o[c][f + 4];
+        }
+        face_inv(nb[c]);
+        own0[c] = nb[c][0];
+      }
+#pragma unroll
+      for (int f = 1; f < 4; ++f)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) yo[(f - 1) * 3 + c][tid] = nb[c][f];
     }
-#pragma unroll
-    for (int f = 1; f < 4; ++f)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) yo[f - 1][c][tid] = nb[c][f];
     if (diag_fly) ss[tid] = se;
     __syncthreads();
+    // MFMA form: node (ly, lx) of plane ez sums the bottom corners m = 2 oy + ox
+    // of elements (ly - oy, lx - ox) of this layer and the top corners of the
+    // previous layer's (carried), and carries this layer's top corners
+    T kmf[3] = {T(0), T(0), T(0)};
+    if constexpr (MF) {
+      if (ly >= 1 && lx >= 1) {
+        const int t1 = tid - kTile, t2 = tid - 1, t3 = tid - 1 - kTile;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          kmf[c] = top_prev[c] + yo[c][tid] + yo[3 + c][t1] + yo[6 + c][t2] + yo[9 + c][t3];
+          top_prev[c] = yo[12 + c][tid] + yo[15 + c][t1] + yo[18 + c][t2] + yo[21 + c][t3];
+        }
+      }
+    }
     if (diag_fly && owner) {
       // diag(K) at node (ly, lx) of plane ez: kd[c] times the scales of the 8
       // elements around it (4 in layer ez - 1, carried in s_prev, 4 in layer ez)
@@ -420,7 +507,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
       const int tym = tid - 1, txm = tid - kTile, tbm = tid - kTile - 1;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T ku = nb[c][0] + yo[1][c][tym] + yo[0][c][txm] + yo[2][c][tbm];
+        const T ku = MF ? kmf[c] : own0[c] + yo[3 + c][tym] + yo[c][txm] + yo[6 + c][tbm];
         const int i = 3 * n_out + c;
         const bool fx = (fix_prev >> c) & 1;
         // the node operand (kModeChebZero: x1 = c0 D^-1 b, formed again as in `store`)
@@ -483,15 +570,15 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
 #define TK_EL_ARGS                                                                              \
   Grid g, SparseKE<float> Kv, const T* __restrict__ Kg, const T* __restrict__ s,                \
       const uint8_t* __restrict__ fixed, const T* __restrict__ u, Epilogue<T> ep, int nty, int zc, \
-      int with_dot, const int* __restrict__ done, double* partials
-template <typename T, int MODE>
+      int with_dot, const int* __restrict__ done, double* partials, const uint4* __restrict__ kemf
+template <typename T, int MODE, bool MF = false>
 __global__ __launch_bounds__(256) void k_apply_el(TK_EL_ARGS) {
-  apply_el<T, MODE>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials);
+  apply_el<T, MODE, MF>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials, kemf);
 }
-template <typename T, int MODE>
+template <typename T, int MODE, bool MF = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_apply_el_w3(
     TK_EL_ARGS) {
-  apply_el<T, MODE>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials);
+  apply_el<T, MODE, MF>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials, kemf);
 }
 #undef TK_EL_ARGS
 
@@ -565,6 +652,28 @@ bool operator_setup(Plan& P) {
     P.sec64[k] = sec[k];
     P.sec32[k] = (float)sec[k];
   }
+  // MFMA form: KE' (rows / columns 4 m + r, corner m = 4 oz + 2 oy + ox, r = 3 a
+  // zero pad) in BF16, laid out per lane as the A operand of the 32x32x16
+  // product takes it: lane l, K step t holds row l % 32, columns 16 t + 8 (l / 32) + 0..7
+  {
+    int ref_corner[8];
+    for (int a = 0; a < 8; ++a)
+      ref_corner[4 * corner_z(a) + 2 * corner_y(a) + corner_x(a)] = a;
+    auto kep = [&](int i, int k) -> double {
+      const int mi = i >> 2, ri = i & 3, mk = k >> 2, rk = k & 3;
+      if (ri == 3 || rk == 3) return 0.0;
+      return P.KE[(3 * ref_corner[mi] + ri) * 24 + 3 * ref_corner[mk] + rk];
+    };
+    std::vector<__bf16> lanes(64 * 2 * 8);
+    for (int l = 0; l < 64; ++l)
+      for (int t = 0; t < 2; ++t)
+        for (int q = 0; q < 8; ++q)
+          lanes[(l * 2 + t) * 8 + q] = (__bf16)(float)kep(l % 32, 16 * t + 8 * (l / 32) + q);
+    if (hipMalloc(&P.kemf, lanes.size() * sizeof(__bf16)) != hipSuccess ||
+        hipMemcpy(P.kemf, lanes.data(), lanes.size() * sizeof(__bf16), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      P.kemf = nullptr;
+  }
   if (hipMalloc((void**)&P.sec64_dev, sizeof(double) * kSecN) != hipSuccess ||
       hipMalloc((void**)&P.sec32_dev, sizeof(float) * kSecN) != hipSuccess ||
       hipMemcpy(P.sec64_dev, P.sec64, sizeof(double) * kSecN, hipMemcpyHostToDevice) != hipSuccess ||
@@ -609,41 +718,57 @@ static void el_grid(const Grid& g, int slots, long long max_blocks, int& nty, in
 }
 
 // resident workgroups per compute unit of one element-kernel instantiation
-template <typename T, int MODE>
+template <typename T, int MODE, bool MF>
 static int el_resident() {
   static int v = 0;
   if (v == 0) {
     constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
     int n = 0;
-    const void* fn = w3 ? (const void*)k_apply_el_w3<T, w3 ? MODE : kModeApply>
-                        : (const void*)k_apply_el<T, w3 ? kModeResidual : MODE>;
+    const void* fn = w3 ? (const void*)k_apply_el_w3<T, w3 ? MODE : kModeApply, MF>
+                        : (const void*)k_apply_el<T, w3 ? kModeResidual : MODE, MF>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
     v = n;
   }
   return v;
 }
 
-// Returns the number of workgroups (= partials per reduced quantity).
-template <typename T, int MODE>
-static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
-                     int with_dot, const int* done, hipStream_t s, bool finalize_alpha = true) {
+template <typename T, int MODE, bool MF>
+static int launch_el_as(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
+                        int with_dot, const int* done, hipStream_t s) {
   int nty, zc;
   dim3 grid;
   // the reduction scratch holds 4 values per block of the largest 1-D launch
-  el_grid(L.g, P.ncu * el_resident<T, MODE>(), P.partials_len / 4, nty, zc, grid);
+  el_grid(L.g, P.ncu * el_resident<T, MODE, MF>(), P.partials_len / 4, nty, zc, grid);
   SparseKE<float> kv;
   for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
+  const uint4* km = (const uint4*)P.kemf;
   constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
   if (w3)
-    hipLaunchKernelGGL((k_apply_el_w3<T, w3 ? MODE : kModeApply>), grid, dim3(256), 0, s, L.g, kv,
-                       sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done, P.partials);
-  else
-    hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE>), grid, dim3(256), 0, s, L.g,
+    hipLaunchKernelGGL((k_apply_el_w3<T, w3 ? MODE : kModeApply, MF>), grid, dim3(256), 0, s, L.g,
                        kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
-                       P.partials);
-  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot && finalize_alpha)
-    finalize(P, kFinAlpha, (int)(grid.x * grid.y), s);
+                       P.partials, km);
+  else
+    hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE, MF>), grid, dim3(256), 0, s, L.g,
+                       kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
+                       P.partials, km);
   return (int)(grid.x * grid.y);
+}
+
+// Returns the number of workgroups (= partials per reduced quantity).  FP32
+// sweeps take the BF16 MFMA form when the plan asks for it (operator_kind 2).
+template <typename T, int MODE>
+static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
+                     int with_dot, const int* done, hipStream_t s, bool finalize_alpha = true) {
+  int G;
+  if constexpr (sizeof(T) == 4 && MODE != kModeApplyDir) {
+    G = P.mfma ? launch_el_as<T, MODE, true>(P, L, sl, u, ep, with_dot, done, s)
+               : launch_el_as<T, MODE, false>(P, L, sl, u, ep, with_dot, done, s);
+  } else {
+    G = launch_el_as<T, MODE, false>(P, L, sl, u, ep, with_dot, done, s);
+  }
+  if ((MODE == kModeApply || MODE == kModeApplyDir) && with_dot && finalize_alpha)
+    finalize(P, kFinAlpha, G, s);
+  return G;
 }
 
 // Flexible-CG direction and operator in one sweep (see kModeApplyDir):

@@ -37,7 +37,8 @@ struct Options {
   int max_levels = kMaxLevels;
   int coarse_max_dof = kCoarseMaxDof;
   int check_every = 8;           // Krylov convergence is read back every N iterations
-  int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node kernel
+  int operator_kind = 0;         // 0: element/sector kernel when KE allows it, 1: dense node
+                                 // kernel, 2: element kernel with BF16 MFMA FP32 sweeps
   int no_graphs = 0;             // 1: no captured solver chunks
   int filter_kind = 0;           // 0: LDS tiles when the radius allows, 1: stencil gather
   int direction_slots = 0;       // mixed-solver direction ring (0: by memory, else 2..64)
@@ -62,6 +63,8 @@ struct Plan {
   double lam_bound = 0.0;         // lambda_max(diag(KE)^-1 KE): bound of lambda_max(D^-1 K)
   bool sector_ok = false;         // KE block-diagonalises in the Walsh-Hadamard basis
   bool use_el = false;            // element kernel in use
+  bool mfma = false;              // FP32 element sweeps in the BF16 MFMA form (operator_kind 2)
+  void* kemf = nullptr;           // its KE' operand, BF16, per-lane layout (2 KiB)
   double sec64[45];               // non-zero sector entries of W^T KE W / 64
   float sec32[45];
   double kd[3] = {0, 0, 0};       // KE diagonal per component
