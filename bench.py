@@ -37,6 +37,13 @@ METRIC = "design iters/s at 256x128x128 H8: mixed vs FP64 GPU, c err <=1e-6 vs F
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+# arithmetic operations the Walsh-Hadamard sector form executes per element and
+# element layer (24 + 21 + 45 + 21 + 21 + 12 + 24), against 1152 flops of the
+# dense 24x24 product the "dense_equiv" figures count
+SECTOR_OPS = 168
+# VALU lane operations per second without packed FP32: 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz (an FMA counts as one operation)
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
 
 def _parse():
@@ -50,7 +57,13 @@ def _parse():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="CPU seconds per sample size of the CPU baseline (two sizes)")
+    ap.add_argument("--no-operator-rates", action="store_true",
+                    help="skip the K u applications/s at every benchmark grid")
+    ap.add_argument("--config4", action="store_true",
+                    help="run the 320x160x160 combined configuration (volfrac 0.2, rmin 4, "
+                         "beta 1, 2, 4, 8 x 40 iterations) on both paths instead")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--smooth-degree", type=int, default=0,
                     help="Chebyshev degree on the two finest levels (0: default)")
@@ -153,14 +166,15 @@ def _time_kernels(problem, st, torch, args):
     y32 = torch.empty_like(u32)
     u64 = u32.double()
     y64 = torch.empty_like(u64)
+    mf = ", true" if plan.options["operator_kind"] == 2 else ""
     plan.prepare(st.rho, 3.0, precision="mixed")
     t = timed(lambda: plan.apply_K(None, 3.0, u32, y32, precision="mixed"))
     out["operator_fp32"] = {"s": t, "bytes": 4 * (2 * ndof + nele), "flops": 1152 * nele,
-                            "kernel": "k_apply_el_w3<float, 0>"}
+                            "ops": SECTOR_OPS * nele, "kernel": f"k_apply_el_w3<float, 0{mf}>"}
     plan.prepare(st.rho, 3.0, precision="fp64")
     t = timed(lambda: plan.apply_K(None, 3.0, u64, y64, precision="fp64"))
     out["operator_fp64"] = {"s": t, "bytes": 8 * (2 * ndof + nele), "flops": 1152 * nele,
-                            "kernel": "k_apply_el_w3<double, 0>"}
+                            "ops": SECTOR_OPS * nele, "kernel": "k_apply_el_w3<double, 0>"}
     # the dominant kernel of the mixed solve: the FP32 Chebyshev smoothing step
     plan.prepare(st.rho, 3.0, precision="mixed")
     b32 = torch.randn(ndof, dtype=torch.float32, device=dev)
@@ -170,11 +184,57 @@ def _time_kernels(problem, st, torch, args):
     # reads x, b, d and the element scales (the inverse diagonal is formed from
     # the scales), writes x' and d
     out["smoother_fp32"] = {"s": t, "bytes": 4 * (5 * ndof + nele), "flops": 1152 * nele,
-                            "kernel": "k_apply_el<float, 3>"}
+                            "ops": SECTOR_OPS * nele, "kernel": f"k_apply_el<float, 3{mf}>"}
     rf, rho, d = problem.new_elem(), problem.new_elem(), problem.new_elem()
     t = timed(lambda: plan.filter_project(st.x, 1.0, 0.5, rf, rho, d))
     out["filter_project"] = {"s": t, "bytes": nele * (8 + 8 + 1 + 24),
-                             "flops": 2 * 93 * nele, "kernel": "k_filter_tiled<0, 2>"}
+                             "flops": 2 * 93 * nele, "ops": 2 * 93 * nele,
+                             "kernel": "k_filter_tiled<0, 2>"}
+    return out
+
+
+# Benchmark configurations whose operator applications/s are reported
+# (plain 60x30x20, 128x64x64, 96x48x48, the headline 256x128x128, 320x160x160).
+OPERATOR_GRIDS = [(60, 30, 20), (128, 64, 64), (96, 48, 48), (256, 128, 128), (320, 160, 160)]
+
+
+def _operator_rates(args):
+    """K u applications/s (FP64 and FP32, one full-grid launch each, HIP
+    events) on a seeded uniform [0.01, 1] design at each benchmark grid, with
+    the HBM fraction of the algorithmic bytes 8 (2 ndof + nele) / 4 (...)."""
+    import torch
+    from pytopo3d_amd import _native
+    from pytopo3d_amd.problem import element_flags, element_stiffness, support_node_flags
+    KE = element_stiffness(None, 1.0)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = {}
+    for nelx, nely, nelz in OPERATOR_GRIDS:
+        plan = _native.Plan(nelx, nely, nelz, KE, 1.5,
+                            {"operator_kind": args.operator_kind} if args.operator_kind else None)
+        z = np.zeros((nely, nelx, nelz), bool)
+        plan.set_masks(torch.from_numpy(element_flags(z, z)).to(dev),
+                       torch.from_numpy(support_node_flags(nelx, nely, nelz)).to(dev))
+        rho = torch.from_numpy(np.random.default_rng(7).uniform(0.01, 1.0, plan.nele)).to(dev)
+        row = {}
+        for prec, dt, nb in (("fp64", torch.float64, 8), ("mixed", torch.float32, 4)):
+            u = torch.randn(plan.ndof, dtype=dt, device=dev)
+            y = torch.empty_like(u)
+            plan.prepare(rho, 3.0, precision=prec)
+            plan.apply_K(None, 3.0, u, y, precision=prec)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.kernel_reps):
+                plan.apply_K(None, 3.0, u, y, precision=prec)
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
+            key = "fp64" if prec == "fp64" else "fp32"
+            row[key] = {"applies_per_s": 1.0 / t, "us": t * 1e6,
+                        "hbm_frac": nb * (2 * plan.ndof + plan.nele) / t / 1e9 / HBM_PEAK_GBS}
+        out[f"{nelx}x{nely}x{nelz}"] = row
+        plan.close()
+        torch.cuda.empty_cache()
     return out
 
 
@@ -227,12 +287,11 @@ def _traffic(kernel_prefix):
         return None
 
 
-def _cpu_baseline(args):
-    """Oracle design iterations (assembled K + SuperLU + sparse filter + OC) on
-    this host at 30x15x10, scaled per element to the 256x128x128 workload."""
+def _cpu_rate(nelx, nely, nelz, seconds):
+    """Design iterations/s of the oracle (COO->CSR assembly, SuperLU, sparse
+    filter, OC bisection) on one host core at one grid, beta 1."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
-    nelx, nely, nelz = 30, 15, 10
     F = oracle.edge_load_force_vector(nelx, nely, nelz)
     solid, void, free = oracle.boundary_protection(nelx, nely, nelz, None, None, None, None, F)
     KE = oracle.element_stiffness(None, 1.0)
@@ -255,21 +314,80 @@ def _cpu_baseline(args):
         x, _ = oracle.oc_projected(x, dcdx, dvdx, 0.2, target, H, Hs, 1.0, 0.5, free, solid, void)
         _, rho, d = oracle.physical_density(x, H, Hs, 1.0, 0.5, solid, void)
         n += 1
-        if time.perf_counter() - t0 >= args.cpu_seconds:
+        if time.perf_counter() - t0 >= seconds:
             break
-    dt = time.perf_counter() - t0
-    rate_sample = n / dt
-    scale = (nelx * nely * nelz) / float(args.nelx * args.nely * args.nelz)
+    return n, time.perf_counter() - t0
+
+
+def _cpu_baseline(args):
+    """The oracle's design iteration (the reference's CPU path with SuperLU,
+    PyPardiso not being installed) measured on this host at two grids; the
+    cost exponent fitted to the two measurements carries the rate to the
+    benchmark grid, where the assembled problem alone would need ~77 GB."""
+    pts = []
+    for dims in ((20, 10, 7), (30, 15, 10)):
+        n, dt = _cpu_rate(*dims, args.cpu_seconds)
+        pts.append((dims, n, dt))
+    (d1, n1, t1), (d2, n2, t2) = pts
+    e1, e2 = np.prod(d1), np.prod(d2)
+    s1, s2 = t1 / n1, t2 / n2                          # seconds per design iteration
+    alpha = float(np.log(s2 / s1) / np.log(e2 / e1))
+    target = args.nelx * args.nely * args.nelz
+    value = 1.0 / (s2 * (target / e2) ** alpha)
     return {
-        "value": rate_sample * scale,
+        "value": value,
         "unit": "design iters/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{n} oracle design iterations (SuperLU direct solve, single-threaded) at "
-                   f"{nelx}x{nely}x{nelz} in {dt:.1f} s = {rate_sample:.3f} iters/s, scaled "
-                   f"linearly by element count to {args.nelx}x{args.nely}x{args.nelz} "
-                   "(optimistic: a 3-D direct solve grows faster than linearly)"),
+        "sample": (f"oracle design iterations (assembled K + SuperLU, single core): "
+                   f"{n1} at {'x'.join(map(str, d1))} in {t1:.1f} s ({1 / s1:.3f} iters/s), "
+                   f"{n2} at {'x'.join(map(str, d2))} in {t2:.1f} s ({1 / s2:.3f} iters/s); "
+                   f"time per iteration grows as elements^{alpha:.2f} between them, which "
+                   f"carries the rate to {args.nelx}x{args.nely}x{args.nelz}"),
+        "measured": [{"grid": list(d), "iters": n, "seconds": t} for d, n, t in pts],
+        "exponent": alpha,
     }
+
+
+def _config4(args):
+    """320x160x160 combined configuration: volfrac 0.2, penal 3, rmin 4, beta 1,
+    2, 4, 8 with 40 design iterations each, edge-load cantilever, through
+    top3d on the mixed path (FP32 multigrid inside FP64 flexible CG, rtol 1e-8)
+    and the FP64 path; design iterations/s, final compliance of each, peak
+    device memory."""
+    import torch
+    from pytopo3d_amd import edge_load_force_vector, top3d
+    nelx, nely, nelz = 320, 160, 160
+    F = edge_load_force_vector(nelx, nely, nelz)
+    kw = dict(volfrac=0.2, penal=3.0, rmin=4.0, disp_thres=0.5, force_vector=F, maxloop=40,
+              tolx=1e-12, beta_schedule=(1.0, 2.0, 4.0, 8.0), projection_eta=0.5, move=0.2,
+              solver_rtol=args.rtol)
+    out = {"config": "320x160x160, volfrac 0.2, rmin 4, beta 1,2,4,8 x 40 iterations"}
+    for prec in ("mixed", "fp64"):
+        torch.cuda.synchronize()
+        free0, total = torch.cuda.mem_get_info()
+        low = [free0]
+        tm = {}
+
+        def cb(d):
+            if d["total"] % 20 == 0:
+                low.append(torch.cuda.mem_get_info()[0])
+                _log(args, f"config4 {prec} iteration {d['total']}: c={d['c']:.6e}")
+
+        t0 = time.perf_counter()
+        _, _, c, _ = top3d(nelx, nely, nelz, precision=prec, timings=tm, iteration_callback=cb,
+                           **kw)
+        dt = time.perf_counter() - t0
+        iters = len(tm["solve_iters"])
+        out[prec] = {"design_iters": iters, "loop_s": tm["iter_s"],
+                     "design_iters_per_s": iters / tm["iter_s"], "wall_s": dt,
+                     "final_compliance": c, "cg_iters_mean": float(np.mean(tm["solve_iters"])),
+                     "peak_device_gb": (free0 - min(low)) / 1e9}
+    out["c_rel_diff_mixed_vs_fp64"] = abs(out["mixed"]["final_compliance"] -
+                                          out["fp64"]["final_compliance"]) / abs(
+        out["fp64"]["final_compliance"])
+    out["speedup"] = out["mixed"]["design_iters_per_s"] / out["fp64"]["design_iters_per_s"]
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -280,6 +398,9 @@ def main():
         sys.exit(2)
     if args.kernel_probe:
         _kernel_probe(args)
+        return
+    if args.config4:
+        _config4(args)
         return
     problem, x0, target, DesignState, torch = _setup(args)
     plan = problem.plan
@@ -345,12 +466,19 @@ def main():
         "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(dom["kernel"]),
         "kernel": dom["kernel"],
         "launch_us": dom["s"] * 1e6,
-        "flop_frac_fp32": dom["flops"] / dom["s"] / 1e12 / FP32_PEAK_TFLOPS,
+        # executed sector-form operations per second as a fraction of the VALU
+        # lane-operation rate, and the dense-equivalent 24x24 rate (not executed)
+        "executed_ops_frac_valu": dom["ops"] / dom["s"] / VALU_LANE_OPS,
+        "dense_equiv_flop_frac_fp32": dom["flops"] / dom["s"] / 1e12 / FP32_PEAK_TFLOPS,
     }
     result["kernels"] = {k: {"kernel": v["kernel"], "us": v["s"] * 1e6,
                              "GBps": v["bytes"] / v["s"] / 1e9,
                              "hbm_frac": v["bytes"] / v["s"] / 1e9 / HBM_PEAK_GBS,
-                             "TFLOPs": v["flops"] / v["s"] / 1e12} for k, v in kern.items()}
+                             "executed_Tops": v["ops"] / v["s"] / 1e12,
+                             "dense_equiv_TFLOPs": v["flops"] / v["s"] / 1e12}
+                         for k, v in kern.items()}
+    if not args.no_operator_rates:
+        result["operator_applies"] = _operator_rates(args)
     if not args.no_cpu:
         result["cpu_baseline"] = _cpu_baseline(args)
     else:
