@@ -121,3 +121,32 @@ def test_design_iterations_mixed_vs_fp64_256x128x128():
         assert a["change"] == pytest.approx(b["change"], abs=1e-6)
     assert np.abs(xm - xd).max() <= 1e-6
     pr.close()
+
+
+def test_fifty_design_iterations_mixed_vs_fp64_256x128x128():
+    """The headline grid's 50 full design iterations (beta 1 then 2, 25 each,
+    the edge-load cantilever, volfrac 0.3, rmin 3) through top3d on the mixed
+    path (FP32 multigrid inside FP64 flexible CG, rtol 1e-8) and the FP64 path
+    (rtol 1e-10): compliance per iteration within 1e-6, designs within 1e-5 and
+    the final compliances within 1e-6."""
+    nelx, nely, nelz = 256, 128, 128
+    note("50 design iterations 256x128x128")
+    F = edge_load_force_vector(nelx, nely, nelz)
+    kw = dict(volfrac=0.3, penal=3.0, rmin=3.0, disp_thres=0.5, force_vector=F, maxloop=25,
+              tolx=1e-12, beta_schedule=(1.0, 2.0), save_history=True, history_frequency=25)
+    from pytopo3d_amd import top3d
+    runs = {}
+    for prec, rtol in (("mixed", 1e-8), ("fp64", 1e-10)):
+        cs = []
+        rho, hist, c, _ = top3d(nelx, nely, nelz, precision=prec, solver_rtol=rtol,
+                                iteration_callback=lambda d: cs.append(d["c"]), **kw)
+        runs[prec] = (np.asarray(cs), rho, c, hist)
+    (cm, rm, fm, hm), (cd, rd, fd, hd) = runs["mixed"], runs["fp64"]
+    assert cm.size == cd.size == 50
+    err = float(np.max(np.abs(cm - cd) / np.abs(cd)))
+    note("  max rel c diff per iteration", err, "| final c", fm, fd,
+         "| max design diff", float(np.abs(rm - rd).max()))
+    assert err <= 1e-6
+    assert np.abs(rm - rd).max() <= 1e-5
+    assert fm == pytest.approx(fd, rel=1e-6)
+    assert hm["iteration_history"] == hd["iteration_history"]
