@@ -6,9 +6,11 @@
 // zero padding outside the domain -- padding with zeros is exactly the
 // boundary truncation of pytopo3d/utils/filter.py:122-228.  Each thread owns
 // one (y, x) column of 8 outputs along z.  The stencil is grouped into (dy, dx)
-// columns; a column with z half-width L is read once from LDS (8 + 2L values)
-// and applied to all 8 outputs from registers, which cuts the LDS reads per
-// output from 93 to about 34 at rmin = 3.
+// columns, and columns with the same z weights (for the distance-based weights
+// of the reference, those with the same dy^2 + dx^2) form one group: the
+// group's source columns are summed first (8 + 2L values each, read once from
+// LDS), then its 2L + 1 weights are applied to the sum from registers.  At
+// rmin = 3 the 93 products per output become 24 products and 25 additions.
 #include "tk_filter_dev.hpp"
 #include "tk_plan.hpp"
 
@@ -28,20 +30,37 @@ constexpr int FMAXW = 343;        // weights of all columns, (2*3+1)^3 at most
 
 struct FilterCols {
   int H;
-  int ncol[FHMAX + 1];            // columns per half-width L = 0..3 (grouped in this order)
-  int dydx[FMAXCOL];              // (dy + 16) | (dx + 16) << 8
-  int woff[FMAXCOL];              // offset of the column's 2L+1 weights in w
+  int ngrp[FHMAX + 1];            // column groups per half-width L = 0..3 (stored in this order)
+  int gfirst[FMAXCOL];            // a group's first member in dydx
+  int gcount[FMAXCOL];            // its member count
+  int woff[FMAXCOL];              // offset of its 2L+1 weights in w
+  int dydx[FMAXCOL];              // members, group by group: (dy + 16) | (dx + 16) << 8
   double w[FMAXW];
 };
 
 enum { kFProject = 0, kFSum = 1, kFOcVolume = 2, kFPlain = 3 };
 
-template <int L>
-__device__ __forceinline__ void col_accum(const double* __restrict__ tile, int base, int zs,
-                                          const double* __restrict__ w, double acc[FTZ]) {
+// One column group: the members' source columns summed in member order, then
+// the group's z weights applied to the sum.  base: the thread's tile offset
+// at the first source plane; the members' (dy, dx) are uniform.
+template <int L, int TYH>
+__device__ __forceinline__ void grp_accum(const double* __restrict__ tile, int base, int zs,
+                                          const FilterCols* __restrict__ fc, int gi,
+                                          double acc[FTZ]) {
   double v[FTZ + 2 * L];
+  const int m0 = fc->gfirst[gi], mn = fc->gcount[gi];
+  for (int m = 0; m < mn; ++m) {
+    const int dd = fc->dydx[m0 + m];
+    const int off = base + ((dd >> 8) & 255) * TYH + (dd & 255) - 16 * (TYH + 1);
+    if (m == 0) {
 #pragma unroll
-  for (int k = 0; k < FTZ + 2 * L; ++k) v[k] = tile[base + k * zs];
+      for (int k = 0; k < FTZ + 2 * L; ++k) v[k] = tile[off + k * zs];
+    } else {
+#pragma unroll
+      for (int k = 0; k < FTZ + 2 * L; ++k) v[k] += tile[off + k * zs];
+    }
+  }
+  const double* w = fc->w + fc->woff[gi];
 #pragma unroll
   for (int j = 0; j <= 2 * L; ++j) {
     const double wj = w[j];
@@ -58,13 +77,6 @@ __global__ __launch_bounds__(FTHREADS) void k_filter_tiled(
     double* __restrict__ o2, double* partials, Scalars* sc) {
   if (MODE == kFOcVolume && sc->oc_done) return;
   extern __shared__ double tile[];
-  __shared__ FilterCols cols;
-  {
-    const int* a = reinterpret_cast<const int*>(colsg);
-    int* b = reinterpret_cast<int*>(&cols);
-    for (int i = threadIdx.x; i < (int)(sizeof(FilterCols) / sizeof(int)); i += blockDim.x)
-      b[i] = a[i];
-  }
   constexpr int TYH = FTY + 2 * H, TXH = FTX + 2 * H, TZH = FTZ + 2 * H;
   constexpr int zs = TYH * TXH;
   // XCD-aware order: z, then y, then x neighbours of a tile share an L2
@@ -72,42 +84,47 @@ __global__ __launch_bounds__(FTHREADS) void k_filter_tiled(
   const int bz = lg % gridDim.z, rr = lg / gridDim.z;
   const int y0 = (rr % gridDim.x) * FTY, x0 = (rr / gridDim.x) * FTX, z0 = bz * FTZ;
   const int exy = g.nelx * g.nely;
-  constexpr int ntile = zs * TZH;
-  // all global loads of the tile are issued before the first LDS store, so
-  // their latencies overlap instead of adding up
-  constexpr int NT = (ntile + FTHREADS - 1) / FTHREADS;
-  double stg[NT];
+  // staging: each thread takes one or two (y, x) positions of the padded tile
+  // and its column of TZH planes; all global loads are issued before the
+  // first LDS store, so their latencies overlap instead of adding up
+  constexpr int NP = (zs + FTHREADS - 1) / FTHREADS;
+  double stg[NP][TZH];
+  bool zin[TZH];
 #pragma unroll
-  for (int k = 0; k < NT; ++k) {
-    const int t = threadIdx.x + FTHREADS * k;
-    const int yy = t % TYH, r = t / TYH, xx = r % TXH, zz = r / TXH;
-    const int ey = y0 - H + yy, ex = x0 - H + xx, ez = z0 - H + zz;
-    stg[k] = (t < ntile && (unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx &&
-              (unsigned)ez < (unsigned)g.nelz)
-                 ? src[ey + ex * g.nely + ez * exy]
-                 : 0.0;
+  for (int zz = 0; zz < TZH; ++zz) zin[zz] = (unsigned)(z0 - H + zz) < (unsigned)g.nelz;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = threadIdx.x + FTHREADS * j;
+    const int ey = y0 - H + p % TYH, ex = x0 - H + p / TYH;
+    const bool in = p < zs && (unsigned)ey < (unsigned)g.nely && (unsigned)ex < (unsigned)g.nelx;
+    const double* col = src + (in ? ey + ex * g.nely + (long long)(z0 - H) * exy : 0);
+#pragma unroll
+    for (int zz = 0; zz < TZH; ++zz)
+      stg[j][zz] = (in && zin[zz]) ? col[(long long)zz * exy] : 0.0;
   }
 #pragma unroll
-  for (int k = 0; k < NT; ++k)
-    if (threadIdx.x + FTHREADS * k < ntile) tile[threadIdx.x + FTHREADS * k] = stg[k];
+  for (int j = 0; j < NP; ++j) {
+    const int p = threadIdx.x + FTHREADS * j;
+    if (p < zs)
+#pragma unroll
+      for (int zz = 0; zz < TZH; ++zz) tile[p + zz * zs] = stg[j][zz];
+  }
   __syncthreads();
   const int ty = threadIdx.x % FTY, tx = threadIdx.x / FTY;
   double acc[FTZ];
 #pragma unroll
   for (int z = 0; z < FTZ; ++z) acc[z] = 0.0;
-  int c = 0;
-#define TK_COLS(LL)                                                                      \
+  // the table is read with uniform (scalar) loads; groups are stored by L
+  int gi = 0;
+#define TK_GROUPS(LL)                                                                    \
   if (H >= LL)                                                                            \
-    for (int i = 0; i < cols.ncol[LL]; ++i, ++c) {                                        \
-      const int dy = (cols.dydx[c] & 255) - 16, dx = ((cols.dydx[c] >> 8) & 255) - 16;    \
-      const int base = (H - LL) * zs + (tx + H + dx) * TYH + (ty + H + dy);               \
-      col_accum<LL>(tile, base, zs, cols.w + cols.woff[c], acc);                          \
-    }
-  TK_COLS(0)
-  TK_COLS(1)
-  TK_COLS(2)
-  TK_COLS(3)
-#undef TK_COLS
+    for (int i = 0; i < colsg->ngrp[LL]; ++i, ++gi)                                       \
+      grp_accum<LL, TYH>(tile, (H - LL) * zs + (tx + H) * TYH + (ty + H), zs, colsg, gi, acc);
+  TK_GROUPS(0)
+  TK_GROUPS(1)
+  TK_GROUPS(2)
+  TK_GROUPS(3)
+#undef TK_GROUPS
   const int ey = y0 + ty, ex = x0 + tx;
   const bool col_in = ey < g.nely && ex < g.nelx;
   double part[1] = {0.0};
@@ -151,22 +168,27 @@ bool filter_tiled_setup(Plan& P, const std::vector<int>& off, const std::vector<
   if (H > FHMAX || (int)colmap.size() > FMAXCOL) return false;
   FilterCols fc{};
   fc.H = H;
-  std::vector<std::pair<std::pair<int, int>, std::vector<std::pair<int, double>>>> byL[FHMAX + 1];
+  // columns with bitwise equal z weights (hence equal L) share a group
+  std::map<std::vector<double>, std::vector<std::pair<int, int>>> groups[FHMAX + 1];
   for (auto& kv : colmap) {
     auto& v = kv.second;
     std::sort(v.begin(), v.end());
     const int L = -v.front().first;
     if (v.back().first != L || (int)v.size() != 2 * L + 1) return false;
-    byL[L].push_back(kv);
+    std::vector<double> wz;
+    for (auto& e : v) wz.push_back(e.second);
+    groups[L][wz].push_back(kv.first);
   }
-  int c = 0, wo = 0;
+  int gi = 0, mi = 0, wo = 0;
   for (int L = 0; L <= FHMAX; ++L) {
-    fc.ncol[L] = (int)byL[L].size();
-    for (auto& kv : byL[L]) {
-      fc.dydx[c] = (kv.first.first + 16) | ((kv.first.second + 16) << 8);
-      fc.woff[c] = wo;
-      for (auto& e : kv.second) fc.w[wo++] = e.second;
-      ++c;
+    fc.ngrp[L] = (int)groups[L].size();
+    for (auto& kv : groups[L]) {
+      fc.gfirst[gi] = mi;
+      fc.gcount[gi] = (int)kv.second.size();
+      fc.woff[gi] = wo;
+      for (double x : kv.first) fc.w[wo++] = x;
+      for (auto& m : kv.second) fc.dydx[mi++] = (m.first + 16) | ((m.second + 16) << 8);
+      ++gi;
     }
   }
   if (hipMalloc((void**)&P.fcols, sizeof(FilterCols)) != hipSuccess) return false;
