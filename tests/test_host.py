@@ -96,6 +96,39 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(cdll, s)
 
 
+_C_CALLER = r"""
+#include "pytopo3d_hip.h"
+#include <stdio.h>
+int main(void) {
+  tk_options o;
+  tk_options_default(&o);
+  /* a call that fails in argument checking, before any device work */
+  int rc = tk_plan_create(NULL, 4, 4, 4, NULL, 1.5, &o);
+  printf("%d %d %d %s\n", o.smooth_degree, o.coarse_degree, rc, tk_last_error());
+  return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_links_from_c(tmp_path):
+    """A C99 caller includes the header and links the library (no device call)."""
+    lib = os.path.join(ROOT, "pytopo3d_amd", "libpytopo3d_hip.so")
+    if not os.path.exists(lib):
+        from pytopo3d_amd import build
+        build.build(verbose=False)
+    src = tmp_path / "caller.c"
+    src.write_text(_C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(lib)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                    "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", libdir, "-lpytopo3d_hip", "-Wl,-rpath," + libdir],
+                   check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out[:3] == ["2", "7", "1"]
+    assert "out must not be NULL" in " ".join(out[3:])
+
+
 def test_product_refuses_to_run_without_a_gpu(monkeypatch):
     import torch
     monkeypatch.setattr(_native, "_lib", None)
