@@ -76,6 +76,8 @@ def _parse():
                     help="smoothing interval low end on the coarser levels (0: default)")
     ap.add_argument("--cheb-hi", type=float, default=0.0,
                     help="smoothing interval high end, fraction of the lambda_max bound (0: default)")
+    ap.add_argument("--galerkin", type=int, default=0,
+                    help="third multigrid level: 0 default (rediscretised), 1 Galerkin, 2 rediscretised")
     ap.add_argument("--operator-kind", type=int, default=0,
                     help="0: element/sector kernel (default), 1: dense node kernel")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -103,7 +105,7 @@ def _setup(args):
     # are reported as such)
     opts = {"operator_kind": args.operator_kind, "smooth_degree": args.smooth_degree,
             "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo, "cheb_hi": args.cheb_hi,
-            "coarse_cheb_lo": args.coarse_cheb_lo}
+            "coarse_cheb_lo": args.coarse_cheb_lo, "galerkin": args.galerkin}
     problem = SIMPProblem(nelx, nely, nelz, 3.0, KE, F, fixed, solid, void,
                           solver_options={k: v for k, v in opts.items() if v})
     x0 = np.full((nely, nelx, nelz), 0.3)

@@ -85,6 +85,17 @@ typedef struct tk_options {
                              iterations and at the end rather than in every update pass;
                              0 (default): as many as fit in 2 GiB, at most 64.  The result
                              does not depend on it (same sums in the same order) */
+  int galerkin;           /* coarse operator of the third multigrid level (coarse elements of
+                             4x4x4 fine elements): 1 the Galerkin product P^T A P of the
+                             rediscretised second level (exact per coarse element: its 8
+                             children's scales enter separately, not as their mean); 0
+                             (default) or 2 rediscretised like the other levels.  Used when
+                             that level is neither the coarsest nor part of the single-
+                             workgroup tail.  It saves CG iterations on designs with sharp
+                             solid/void contrast inside 4x4x4 blocks, costs ~5 % per
+                             iteration, and left the iteration counts of smooth SIMP designs
+                             (256x128x128, 320x160x160) unchanged; the solve's tolerance is
+                             the same either way */
 } tk_options;
 
 /* Fill *opt with the defaults. */
@@ -144,6 +155,13 @@ int tk_plan_stencil(const tk_plan* plan, int* offsets, double* weights, int cap,
  * entries.  Not in the reference (its filter is a sparse matrix,
  * pytopo3d/utils/filter.py:122-228); it lets tests name the kernel variant. */
 int tk_plan_filter_info(const tk_plan* plan, int* tiled, int* halo, int* nsten);
+/* Galerkin level of the plan's multigrid hierarchy (HOST outputs, each may be
+ * NULL): level = the level whose operator is the Galerkin product of the level
+ * above (-1: none, every level rediscretised), bound = that level's bound of
+ * lambda_max(D^-1 A) used by its Chebyshev smoother (KE's bound when none).
+ * No counterpart in the reference (its solve is a library call,
+ * pytopo3d/utils/solver.py:18-73); it lets tests name the hierarchy. */
+int tk_plan_galerkin_info(const tk_plan* plan, int* level, double* bound);
 /* Copy the filter row sums Hs (device output, nele doubles). */
 int tk_filter_row_sums(tk_plan* plan, double* Hs, void* stream);
 
@@ -169,6 +187,13 @@ int tk_filter_transpose2(tk_plan* plan, const double* g_c, const double* g_v,
  * of pytopo3d/core/optimizer.py:584-591. */
 int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const void* u, void* y,
                void* stream);
+
+/* y = A_l u with multigrid level l's operator as the V-cycle applies it
+ * (identity rows on the level's fixed DOFs), built by the last tk_solve or
+ * tk_prepare in precision prec; u, y: 3 (nx_l + 1)(ny_l + 1)(nz_l + 1) values
+ * (dimensions from tk_plan_levels) of that precision.  No counterpart in the
+ * reference (it has no multigrid); it lets tests check each level's operator. */
+int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, void* stream);
 /* One fine-grid Chebyshev smoothing step of the multigrid preconditioner,
  * with the element scales and inverse diagonal of the last tk_solve in the
  * same precision: d = c1 d + c2 D^-1 (b - K x); xo = x + d (identity rows on

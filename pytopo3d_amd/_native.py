@@ -48,6 +48,7 @@ class TkOptions(ctypes.Structure):
         ("coarse_degree", ctypes.c_int),
         ("coarse_cheb_lo", ctypes.c_double),
         ("direction_slots", ctypes.c_int),
+        ("galerkin", ctypes.c_int),
     ]
 
 
@@ -64,6 +65,8 @@ _SIGS = {
     "tk_plan_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
     "tk_plan_stencil": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D), _I, ctypes.POINTER(_I)]),
     "tk_plan_filter_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "tk_plan_galerkin_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "tk_level_apply": (_I, [_P, _I, _I, _P, _P, _P]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -211,6 +214,17 @@ class Plan:
         _check(lib().tk_plan_filter_info(self._h, ctypes.byref(t), ctypes.byref(h),
                                          ctypes.byref(n)))
         return {"tiled": bool(t.value), "halo": int(h.value), "nsten": int(n.value)}
+
+    def galerkin_info(self) -> dict:
+        """{"level": the Galerkin level (-1: none), "bound": its lambda_max(D^-1 A) bound}."""
+        lv, b = _I(), _D()
+        _check(lib().tk_plan_galerkin_info(self._h, ctypes.byref(lv), ctypes.byref(b)))
+        return {"level": int(lv.value), "bound": float(b.value)}
+
+    def level_apply(self, level: int, u, y, precision="fp64"):
+        """y = A_level u with the hierarchy of the last solve/prepare in this precision."""
+        _check(lib().tk_level_apply(self._h, PRECISIONS[precision], int(level), _ptr(u), _ptr(y),
+                                    _stream()))
 
     def row_sums(self, out: torch.Tensor) -> torch.Tensor:
         _check(lib().tk_filter_row_sums(self._h, _ptr(out), _stream()))

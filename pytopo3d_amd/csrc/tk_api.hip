@@ -27,41 +27,8 @@ void launch_boundary_conditions(Plan& P, const double* ff, const uint8_t* sm, co
                                 const uint8_t* pz, double* F, uint8_t* fixed, uint8_t* flags,
                                 hipStream_t s);
 
-// Largest eigenvalue of diag(KE)^-1/2 KE diag(KE)^-1/2 by cyclic Jacobi rotations.
-static double lambda_bound(const double* KE) {
-  const int n = 24;
-  double a[24][24];
-  double dinv[24];
-  for (int i = 0; i < n; ++i) dinv[i] = 1.0 / std::sqrt(KE[i * 25]);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) a[i][j] = KE[i * n + j] * dinv[i] * dinv[j];
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = i + 1; j < n; ++j) off += a[i][j] * a[i][j];
-    if (off < 1e-30) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        if (std::fabs(a[p][q]) < 1e-300) continue;
-        const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = a[k][p], akq = a[k][q];
-          a[k][p] = c * akp - s * akq;
-          a[k][q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double apk = a[p][k], aqk = a[q][k];
-          a[p][k] = c * apk - s * aqk;
-          a[q][k] = s * apk + c * aqk;
-        }
-      }
-  }
-  double m = 0.0;
-  for (int i = 0; i < n; ++i) m = std::fmax(m, a[i][i]);
-  return m;
-}
+// Largest eigenvalue of diag(KE)^-1/2 KE diag(KE)^-1/2 (cyclic Jacobi rotations).
+static double lambda_bound(const double* KE) { return scaled_lambda_max(KE); }
 
 template <typename T>
 static int dmalloc(T** p, size_t n) {
@@ -86,6 +53,7 @@ static void free_plan(Plan* P) {
   f(P->r32); f(P->p32); f(P->q32); f(P->x32); f(P->pring); f(P->alpha_hist);
   f(P->ework0); f(P->ework1); f(P->ework2); f(P->ework3);
   f(P->partials); f(P->sc);
+  f(P->gal_tab64); f(P->gal_tab32); f(P->gal_dw); f(P->gal_sh64); f(P->gal_sh32);
   if (P->sc_host) (void)hipHostFree(P->sc_host);
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -161,6 +129,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
       return 1;
     }
     P->opt.direction_slots = o->direction_slots;
+    if (o->galerkin >= 0 && o->galerkin <= 2) P->opt.galerkin = o->galerkin;
   }
   if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
     delete P;
@@ -303,6 +272,18 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   }
   if (P->opt.filter_kind == 1 || !filter_tiled_setup(*P, off, w)) P->ftiled = false;
   tail_setup(*P);
+  // Galerkin operator on level 2 when asked for, and when that level is a
+  // level of its own (not in the single-workgroup tail, not the coarsest) and
+  // KE has the expected pattern
+  if (P->opt.galerkin == 1 && P->use_el && P->nlev >= 4 &&
+      (P->tail_level < 0 || P->tail_level > 2) && galerkin_setup(*P)) {
+    P->gal_level = 2;
+    const size_t ne = 8 * (size_t)P->lv[2].g.nele;
+    if (dmalloc(&P->gal_sh64, ne) || dmalloc(&P->gal_sh32, ne)) {
+      free_plan(P);
+      return 2;
+    }
+  }
   launch_filter_hs(*P, s);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -401,6 +382,7 @@ void tk_options_default(tk_options* o) {
   o->coarse_degree = d.coarse_degree;
   o->coarse_cheb_lo = d.coarse_cheb_lo;
   o->direction_slots = d.direction_slots;
+  o->galerkin = d.galerkin;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,
@@ -573,6 +555,38 @@ int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const v
     return 1;
   }
   return tk::apply_fine(P, prec, u, y, s);
+}
+
+int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!u || !y) {
+    set_error("u and y are required");
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  if (level < 0 || level >= P.nlev) {
+    set_error("level must be in [0, " + std::to_string(P.nlev) + ")");
+    return 1;
+  }
+  if (P.last_penal < 0.0 || prec != P.last_prec) {
+    set_error("tk_level_apply needs a previous tk_solve or tk_prepare in this precision");
+    return 1;
+  }
+  return tk::apply_level(P, prec, level, u, y, (hipStream_t)stream);
+}
+
+int tk_plan_galerkin_info(const tk_plan* plan, int* level, double* bound) {
+  if (!plan || !plan->p) {
+    set_error("null plan");
+    return 1;
+  }
+  const tk::Plan& P = *plan->p;
+  if (level) *level = P.gal_level;
+  if (bound) *bound = P.gal_level >= 0 ? P.lam_gal : P.lam_bound;
+  return 0;
 }
 
 int tk_smoother_step(tk_plan* plan, int prec, const void* b, const void* x, void* xo, void* d,

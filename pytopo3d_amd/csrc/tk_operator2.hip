@@ -28,6 +28,7 @@
 // neighbouring tile), exchange the three off-diagonal corner contributions of
 // each node plane through LDS, and every node thread sums its 4 contributions
 // in a fixed order, so results are deterministic.
+#include "tk_galerkin_pattern.hpp"
 #include "tk_plan.hpp"
 #include "tk_sector.hpp"
 
@@ -138,9 +139,14 @@ template <typename T>
 struct LayerIn {
   T se;
   T eb[3], ed[3];
+  T sh[8];   // Galerkin level: the element's Walsh coefficients
+  T ev[3];   // Galerkin level: the node's inverse diagonal (Chebyshev sweeps)
 };
 
-template <typename T, int MODE, bool MF = false>
+// GAL: the Galerkin level (tk_galerkin.hip): s holds 8 Walsh coefficients per
+// element, Kg the coupling table, and the Chebyshev sweeps read D^-1 from
+// ep.dinv (the diagonal is no longer KE's times the scale sum).
+template <typename T, int MODE, bool MF = false, bool GAL = false>
 __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& Kv,
                                          const T* __restrict__ Kg, const T* __restrict__ s,
                                          const uint8_t* __restrict__ fixed,
@@ -148,21 +154,27 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
                                          int zc, int with_dot, const int* __restrict__ done,
                                          double* partials, const uint4* __restrict__ kemf) {
   static_assert(!MF || sizeof(T) == 4, "the MFMA form is an FP32 sweep");
+  static_assert(!(MF && GAL), "the Galerkin level uses the sector-form sweep");
   __shared__ T un[2][3][kPl * kPl];       // two node planes of the tile (masked operand)
   // exchanged corner contributions: sector form (f = 1, 2, 3), or the MFMA
   // form's per-element corner outputs [oz][oy][ox][component]
   __shared__ T yo[MF ? 24 : 9][kTile * kTile];
   __shared__ T kb[kSecN];                 // FP64 sector entries
   __shared__ T ss[kTile * kTile];         // element scales of the layer (on-the-fly diagonal)
+  __shared__ T gl[GAL ? kGalN : 1];       // Galerkin coupling table
   constexpr bool kCheb = MODE == kModeChebFirst || MODE == kModeChebStep || MODE == kModeChebFirstP;
   // Chebyshev sweeps form D^-1 from the element scales (the element kernel
   // is used only when KE's diagonal is the same at every corner)
-  constexpr bool diag_fly = kCheb;
+  constexpr bool diag_fly = kCheb && !GAL;
 
   const int tid = threadIdx.x;
   // FP64 entries are staged in LDS (90 SGPRs would spill); FP32 entries stay
   // in scalar registers.
-  if (sizeof(T) == 8 && tid < kSecN) kb[tid] = Kg[tid];
+  if constexpr (GAL) {
+    for (int i = tid; i < kGalN; i += blockDim.x) gl[i] = Kg[i];
+  } else {
+    if (sizeof(T) == 8 && tid < kSecN) kb[tid] = Kg[tid];
+  }
   const int ly = tid & (kTile - 1), lx = tid >> 4;
   // XCD-aware order: z-chunks of a tile, then y, then x neighbours share an L2
   const int lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -328,9 +340,16 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
                           MODE == kModeChebStep || MODE == kModeChebFirstP;
   auto fetch_layer = [&](int ez, LayerIn<T>& L) {
     const bool el = col_el && (unsigned)ez < (unsigned)g.nelz;
-    L.se = el ? s[ey + ex * g.nely + ez * exy] : T(0);   // 0 gives zero contributions
+    if constexpr (GAL) {
+      const T* q = s + 8 * (size_t)(ey + ex * g.nely + ez * exy);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) L.eb[c] = L.ed[c] = T(0);
+      for (int w = 0; w < 8; ++w) L.sh[w] = el ? q[w] : T(0);
+      L.se = T(0);
+    } else {
+      L.se = el ? s[ey + ex * g.nely + ez * exy] : T(0);   // 0 gives zero contributions
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L.eb[c] = L.ed[c] = L.ev[c] = T(0);
     if constexpr (kNeedB) {
       if (owner && ez >= Z0) {
         const int n_out = ey + ex * g.nny + ez * nxy;
@@ -338,6 +357,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
         for (int c = 0; c < 3; ++c) {
           L.eb[c] = ep.b[3 * n_out + c];
           if (MODE == kModeChebStep) L.ed[c] = ep.dp[3 * n_out + c];
+          if constexpr (GAL && kCheb) L.ev[c] = ep.dinv[3 * n_out + c];
         }
       }
     }
@@ -409,7 +429,7 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       eb[c] = (MODE == kModeChebZero) ? pb_prev[c] : lin.eb[c];
-      edi[c] = (MODE == kModeChebZero) ? pdi_prev[c] : T(0);   // else formed below
+      edi[c] = (MODE == kModeChebZero) ? pdi_prev[c] : (GAL ? lin.ev[c] : T(0));   // else formed below
       ed[c] = lin.ed[c];
     }
     const T se = lin.se;
@@ -454,7 +474,8 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
 #pragma unroll
         for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
       }
-      if constexpr (sizeof(T) == 8) sector_apply<T>(kb, se, a, o);
+      if constexpr (GAL) galerkin_apply<T>(gl, lin.sh, a, o);
+      else if constexpr (sizeof(T) == 8) sector_apply<T>(kb, se, a, o);
       else sector_apply<T>(Kv.v, se, a, o);
       // bottom plane: sum with the previous layer's top, inverse face transform
       T nb[3][4];
@@ -571,9 +592,9 @@ __device__ __forceinline__ void apply_el(const Grid& g, const SparseKE<float>& K
   Grid g, SparseKE<float> Kv, const T* __restrict__ Kg, const T* __restrict__ s,                \
       const uint8_t* __restrict__ fixed, const T* __restrict__ u, Epilogue<T> ep, int nty, int zc, \
       int with_dot, const int* __restrict__ done, double* partials, const uint4* __restrict__ kemf
-template <typename T, int MODE, bool MF = false>
+template <typename T, int MODE, bool MF = false, bool GAL = false>
 __global__ __launch_bounds__(256) void k_apply_el(TK_EL_ARGS) {
-  apply_el<T, MODE, MF>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials, kemf);
+  apply_el<T, MODE, MF, GAL>(g, Kv, Kg, s, fixed, u, ep, nty, zc, with_dot, done, partials, kemf);
 }
 template <typename T, int MODE, bool MF = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_apply_el_w3(
@@ -718,39 +739,50 @@ static void el_grid(const Grid& g, int slots, long long max_blocks, int& nty, in
 }
 
 // resident workgroups per compute unit of one element-kernel instantiation
-template <typename T, int MODE, bool MF>
+// (the Galerkin level's operator-only sweep is the plain kernel, not the
+// 3-wave one)
+template <typename T, int MODE, bool MF, bool GAL = false>
 static int el_resident() {
   static int v = 0;
   if (v == 0) {
-    constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
+    constexpr bool w3 = (MODE == kModeApply || MODE == kModeApplyDir) && !GAL;
     int n = 0;
     const void* fn = w3 ? (const void*)k_apply_el_w3<T, w3 ? MODE : kModeApply, MF>
-                        : (const void*)k_apply_el<T, w3 ? kModeResidual : MODE, MF>;
+                        : (const void*)k_apply_el<T, w3 ? kModeResidual : MODE, MF, GAL>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
     v = n;
   }
   return v;
 }
 
-template <typename T, int MODE, bool MF>
+template <typename T> static const T* gal_table(const Plan& P);
+template <> const double* gal_table<double>(const Plan& P) { return P.gal_tab64; }
+template <> const float* gal_table<float>(const Plan& P) { return P.gal_tab32; }
+template <typename T> static const T* gal_coeffs(const Plan& P);
+template <> const double* gal_coeffs<double>(const Plan& P) { return P.gal_sh64; }
+template <> const float* gal_coeffs<float>(const Plan& P) { return P.gal_sh32; }
+
+// GAL: sl is ignored; the level's Walsh coefficients and the coupling table
+// take the place of its scales and sector entries.
+template <typename T, int MODE, bool MF, bool GAL = false>
 static int launch_el_as(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
                         int with_dot, const int* done, hipStream_t s) {
   int nty, zc;
   dim3 grid;
   // the reduction scratch holds 4 values per block of the largest 1-D launch
-  el_grid(L.g, P.ncu * el_resident<T, MODE, MF>(), P.partials_len / 4, nty, zc, grid);
+  el_grid(L.g, P.ncu * el_resident<T, MODE, MF, GAL>(), P.partials_len / 4, nty, zc, grid);
   SparseKE<float> kv;
   for (int i = 0; i < kSecN; ++i) kv.v[i] = P.sec32[i];
   const uint4* km = (const uint4*)P.kemf;
-  constexpr bool w3 = MODE == kModeApply || MODE == kModeApplyDir;
+  constexpr bool w3 = (MODE == kModeApply || MODE == kModeApplyDir) && !GAL;
+  const T* kg = GAL ? gal_table<T>(P) : sectors_of<T>(P);
+  const T* sc = GAL ? gal_coeffs<T>(P) : sl;
   if (w3)
     hipLaunchKernelGGL((k_apply_el_w3<T, w3 ? MODE : kModeApply, MF>), grid, dim3(256), 0, s, L.g,
-                       kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
-                       P.partials, km);
+                       kv, kg, sc, L.fixed, u, ep, nty, zc, with_dot, done, P.partials, km);
   else
-    hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE, MF>), grid, dim3(256), 0, s, L.g,
-                       kv, sectors_of<T>(P), sl, L.fixed, u, ep, nty, zc, with_dot, done,
-                       P.partials, km);
+    hipLaunchKernelGGL((k_apply_el<T, w3 ? kModeResidual : MODE, MF, GAL>), grid, dim3(256), 0, s,
+                       L.g, kv, kg, sc, L.fixed, u, ep, nty, zc, with_dot, done, P.partials, km);
   return (int)(grid.x * grid.y);
 }
 
@@ -760,6 +792,23 @@ template <typename T, int MODE>
 static int launch_el(Plan& P, const Level& L, const T* sl, const T* u, const Epilogue<T>& ep,
                      int with_dot, const int* done, hipStream_t s, bool finalize_alpha = true) {
   int G;
+  const bool gal = P.gal_level > 0 && &L == &P.lv[P.gal_level];
+  if constexpr (MODE != kModeApplyDir) {
+    if (gal) {
+      // the Galerkin level's smoothing sweeps read D^-1 (and the zero-start
+      // sweep always does): never launch them without it
+      constexpr bool needs_dinv = MODE == kModeChebFirst || MODE == kModeChebStep ||
+                                  MODE == kModeChebFirstP || MODE == kModeChebZero;
+      if (needs_dinv && !ep.dinv) {
+        set_error("internal: Galerkin-level smoothing sweep without an inverse diagonal");
+        P.launch_refused = true;
+        return 0;
+      }
+      G = launch_el_as<T, MODE, false, true>(P, L, sl, u, ep, with_dot, done, s);
+      if (MODE == kModeApply && with_dot && finalize_alpha) finalize(P, kFinAlpha, G, s);
+      return G;
+    }
+  }
   if constexpr (sizeof(T) == 4 && MODE != kModeApplyDir) {
     G = P.mfma ? launch_el_as<T, MODE, true>(P, L, sl, u, ep, with_dot, done, s)
                : launch_el_as<T, MODE, false>(P, L, sl, u, ep, with_dot, done, s);
@@ -834,6 +883,7 @@ void el_cheb(Plan& P, const Level& L, const T* sl, const T* dinv, const T* b, co
   ep.xo = xo;
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
+  ep.dinv = dinv;   // read on the Galerkin level (elsewhere D^-1 is formed from the scales)
   for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.c1 = (T)c1;
@@ -857,6 +907,7 @@ void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const
   ep.xo = xo;
   ep.d = write_d ? d : nullptr;
   ep.dp = d;
+  ep.dinv = dinv;   // read on the Galerkin level
   for (int c = 0; c < 3; ++c) ep.kdi[c] = (T)(1.0 / P.kd[c]);
   ep.c0 = (T)c0;
   ep.xc = xc;

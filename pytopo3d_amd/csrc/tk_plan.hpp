@@ -42,6 +42,7 @@ struct Options {
   int no_graphs = 0;             // 1: no captured solver chunks
   int filter_kind = 0;           // 0: LDS tiles when the radius allows, 1: stencil gather
   int direction_slots = 0;       // mixed-solver direction ring (0: by memory, else 2..64)
+  int galerkin = 0;              // Galerkin operator on the third level: 0 default (off), 1 on, 2 off
 };
 
 struct Counters {
@@ -139,6 +140,16 @@ struct Plan {
   int tail_level = -1;            // level whose V-cycle part runs in one workgroup (-1: none)
   bool tail_inv_lds64 = false;    // the tail keeps a copy of the coarsest inverse in LDS
   bool tail_inv_lds32 = false;
+  // Galerkin operator of one level (tk_galerkin.hip): per element the Walsh
+  // coefficients of its 8 children's scales, the 279-entry coupling table
+  int gal_level = -1;             // the level (-1: every level rediscretised)
+  bool launch_refused = false;    // a sweep was not launched (missing operand): the solve fails
+  double lam_gal = 0.0;           // bound of lambda_max(D^-1 A) on that level
+  double* gal_tab64 = nullptr;    // coupling table (kGalN entries)
+  float* gal_tab32 = nullptr;
+  double* gal_dw = nullptr;       // Walsh-domain corner diagonals (8 x 8 x 3)
+  double* gal_sh64 = nullptr;     // 8 coefficients per element of the level
+  float* gal_sh32 = nullptr;
 };
 
 // Chebyshev coefficients of one smoothing pass on the single-workgroup tail
@@ -248,6 +259,13 @@ void el_cheb_prolong(Plan& P, const Level& L, const Level& C, const T* xc, const
                      const T* dinv, const T* b, const T* x, T* xo, T* d, double c0, int write_d,
                      const int* done, hipStream_t s);
 void launch_coarse_setup(Plan& P, hipStream_t s);
+// Galerkin level (tk_galerkin.hip)
+double scaled_lambda_max(const double* M);
+bool galerkin_setup(Plan& P);
+template <typename T>
+void launch_gal_coeffs(Plan& P, hipStream_t s);
+template <typename T>
+void launch_gal_diag(Plan& P, hipStream_t s);
 // the second-coarsest level's V-cycle part and the coarsest solve in one
 // workgroup (tk_tail.hip); tail_setup picks the level when it fits
 void tail_setup(Plan& P);
@@ -272,6 +290,7 @@ void launch_design_stats(Plan& P, const double* x_old, const double* x_new, cons
 int solve(Plan& P, int prec, const double* rho, double penal, const double* F, double* U,
           double rtol, int maxit, int flags, int* iters_out, double* relres_out, hipStream_t s);
 int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStream_t s);
+int apply_level(Plan& P, int prec, int l, const void* u, void* y, hipStream_t s);
 
 // ---- errors -----------------------------------------------------------------
 void set_error(const std::string& msg);

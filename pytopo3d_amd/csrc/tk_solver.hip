@@ -288,8 +288,10 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
   const bool coarse = l >= 2;
   const int deg = coarse ? P.opt.coarse_degree : P.opt.smooth_degree;
   LevelBuf<T>& B = buf<T>(L);
-  const double lmax = P.lam_bound * P.opt.cheb_hi;
-  const double lmin = P.lam_bound * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
+  // the Galerkin level has its own bound of lambda_max(D^-1 A)
+  const double lam = l == P.gal_level ? P.lam_gal : P.lam_bound;
+  const double lmax = lam * P.opt.cheb_hi;
+  const double lmin = lam * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
@@ -592,6 +594,7 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
   };
   int rc = run_chunks(P, 0, jacobi, maxit, -1, body, s);
   if (rc) return rc;
+  if (P.launch_refused) return 2;   // tk_last_error() names the sweep
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in pcg");
     return 2;
@@ -675,6 +678,7 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
   // iteration 0 ran eagerly: the chunks start at an odd index
   int rc = run_chunks(P, 1, jacobi, maxit, 1, body, s);
   if (rc) return rc;
+  if (P.launch_refused) return 2;   // tk_last_error() names the sweep
   if (hipGetLastError() != hipSuccess) {
     set_error("kernel launch failed in mixed pcg");
     return 2;
@@ -695,6 +699,12 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
     for (int l = 0; l + 1 < P.nlev; ++l)
       launch_coarsen_scale<double>(P.lv[l], P.lv[l + 1], P.lv[l].f64.s, P.lv[l + 1].f64.s, s);
   }
+  // the Galerkin level's coefficients from its children's scales (the
+  // rediscretised scales of that level still feed the levels below it)
+  if (P.gal_level > 0) {
+    if (prec == 0) launch_gal_coeffs<double>(P, s);
+    else launch_gal_coeffs<float>(P, s);
+  }
   // The coarsest factorisation (one workgroup, ~0.4 ms) runs on the plan's
   // set-up stream beside the inverse diagonals, the right-hand side, the
   // initial residual and the first V-cycle's descent; the first coarsest solve
@@ -708,6 +718,10 @@ int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStrea
   // inverse diagonals for the V-cycle (and Jacobi) of this precision only
   if (prec == 0) launch_diag_inv_levels<double>(P, s);
   else launch_diag_inv_levels<float>(P, s);
+  if (P.gal_level > 0) {
+    if (prec == 0) launch_gal_diag<double>(P, s);
+    else launch_gal_diag<float>(P, s);
+  }
   TK_HIP(hipGetLastError());
   P.last_penal = penal;
   P.last_prec = prec;
@@ -751,6 +765,7 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
     if (rc) return rc;
   }
   P.cnt.solves++;
+  P.launch_refused = false;
   double* b = P.lv[0].f64.b;  // masked right-hand side
   // the iterate lives in the plan (x64) so that captured chunks stay valid
   // for every caller buffer; U is copied in (warm start, masked in the same
@@ -799,6 +814,17 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
 }
 
 // Public helper used by the API for a single operator application.
+// y = A_l u with multigrid level l's operator as the V-cycle applies it (the
+// scales, or on the Galerkin level its coefficients, of the last set-up).
+int apply_level(Plan& P, int prec, int l, const void* u, void* y, hipStream_t s) {
+  if (prec == 0)
+    launch_apply<double>(P, P.lv[l], P.lv[l].f64.s, (const double*)u, (double*)y, nullptr, 0, s);
+  else
+    launch_apply<float>(P, P.lv[l], P.lv[l].f32.s, (const float*)u, (float*)y, nullptr, 0, s);
+  TK_HIP(hipGetLastError());
+  return 0;
+}
+
 int apply_fine(Plan& P, int prec, const void* u, void* y, hipStream_t s) {
   if (prec == 0)
     launch_apply<double>(P, P.lv[0], P.lv[0].f64.s, (const double*)u, (double*)y, nullptr, 0, s);
