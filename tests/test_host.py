@@ -212,3 +212,29 @@ def test_preset_helpers_match_reference_semantics():
         build_support_mask(*n, 3, 3, 0, 6, 0, 4)
     m = protected_zone_mask_from_ranges(*n, [(0, 2, 0, 6, 0, 4), (1, 3, 2, 3, 1, 2)])
     assert m.sum() == 2 * 6 * 4 + 1
+
+
+def test_galerkin_pattern_header_matches_its_generator():
+    """tk_galerkin_pattern.hpp is what gen_galerkin.py derives from lk_H8 (the
+    Galerkin coupling pattern, checked there on three orthotropic KEs)."""
+    import importlib.util
+    path = os.path.join(ROOT, "pytopo3d_amd", "csrc", "gen_galerkin.py")
+    spec = importlib.util.spec_from_file_location("gen_galerkin", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    pat = gen.pattern()
+    with open(os.path.join(ROOT, "pytopo3d_amd", "csrc", "tk_galerkin_pattern.hpp")) as fh:
+        text = fh.read()
+
+    def arr(name):
+        m = re.search(name + r"\[[^\]]*\] = \{([^}]*)\}", text)
+        return [int(v) for v in m.group(1).split(",")]
+
+    ws, rows, start, cols = arr("kGalW"), arr("kGalRow"), arr("kGalStart"), arr("kGalCol")
+    got = sorted((ws[g], rows[g], cols[k]) for g in range(len(ws))
+                 for k in range(start[g], start[g + 1]))
+    assert got == pat and len(pat) == 279
+    # the straight-line code applies exactly these entries, in table order
+    terms = re.findall(r"gk\[(\d+)\] \* a\[(\d)\]\[(\d)\]", text)
+    assert [int(k) for k, _, _ in terms] == list(range(279))
+    assert [3 * int(m) + int(c) for _, c, m in terms] == cols
