@@ -194,6 +194,18 @@ int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const v
  * (dimensions from tk_plan_levels) of that precision.  No counterpart in the
  * reference (it has no multigrid); it lets tests check each level's operator. */
 int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, void* stream);
+/* x = the part of the V-cycle that one workgroup runs on the tail level (the
+ * second-coarsest level): pre-smoothing from x = 0, residual, restriction,
+ * coarsest solve with the stored inverse, prolongation and post-smoothing,
+ * with the hierarchy of the last tk_solve or tk_prepare in precision prec.
+ * b (zero on the level's fixed DOFs) and x: 3 (nx_l + 1)(ny_l + 1)(nz_l + 1)
+ * values of that precision; fails when the plan has no tail level.  No
+ * counterpart in the reference (it has no multigrid); it lets tests check the
+ * tail against the level operators of tk_level_apply. */
+int tk_tail_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream);
+/* The tail level of the plan's V-cycle (-1: none; then the coarse levels run
+ * level by level down to the coarsest solve). */
+int tk_plan_tail_level(const tk_plan* plan, int* level);
 /* One fine-grid Chebyshev smoothing step of the multigrid preconditioner,
  * with the element scales and inverse diagonal of the last tk_solve in the
  * same precision: d = c1 d + c2 D^-1 (b - K x); xo = x + d (identity rows on

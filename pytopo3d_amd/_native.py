@@ -67,6 +67,8 @@ _SIGS = {
     "tk_plan_filter_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "tk_plan_galerkin_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "tk_level_apply": (_I, [_P, _I, _I, _P, _P, _P]),
+    "tk_tail_apply": (_I, [_P, _I, _P, _P, _P]),
+    "tk_plan_tail_level": (_I, [_P, ctypes.POINTER(_I)]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -225,6 +227,16 @@ class Plan:
         """y = A_level u with the hierarchy of the last solve/prepare in this precision."""
         _check(lib().tk_level_apply(self._h, PRECISIONS[precision], int(level), _ptr(u), _ptr(y),
                                     _stream()))
+
+    def tail_level(self) -> int:
+        """The level whose V-cycle part runs in one workgroup (-1: none)."""
+        lv = _I()
+        _check(lib().tk_plan_tail_level(self._h, ctypes.byref(lv)))
+        return int(lv.value)
+
+    def tail_apply(self, b, x, precision="fp64"):
+        """x = the single-workgroup V-cycle tail for right-hand side b on the tail level."""
+        _check(lib().tk_tail_apply(self._h, PRECISIONS[precision], _ptr(b), _ptr(x), _stream()))
 
     def row_sums(self, out: torch.Tensor) -> torch.Tensor:
         _check(lib().tk_filter_row_sums(self._h, _ptr(out), _stream()))

@@ -578,6 +578,32 @@ int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, v
   return tk::apply_level(P, prec, level, u, y, (hipStream_t)stream);
 }
 
+int tk_plan_tail_level(const tk_plan* plan, int* level) {
+  if (!plan || !plan->p || !level) {
+    set_error("null plan or level");
+    return 1;
+  }
+  *level = plan->p->tail_level;
+  return 0;
+}
+
+int tk_tail_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!b || !x) {
+    set_error("b and x are required");
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  if (P.last_penal < 0.0 || prec != P.last_prec) {
+    set_error("tk_tail_apply needs a previous tk_solve or tk_prepare in this precision");
+    return 1;
+  }
+  return tk::apply_tail(P, prec, b, x, (hipStream_t)stream);
+}
+
 int tk_plan_galerkin_info(const tk_plan* plan, int* level, double* bound) {
   if (!plan || !plan->p) {
     set_error("null plan");

@@ -270,7 +270,7 @@ void launch_gal_diag(Plan& P, hipStream_t s);
 // workgroup (tk_tail.hip); tail_setup picks the level when it fits
 void tail_setup(Plan& P);
 template <typename T>
-void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s);
+void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s, const int* done);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);
 // energy
@@ -291,6 +291,7 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
           double rtol, int maxit, int flags, int* iters_out, double* relres_out, hipStream_t s);
 int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStream_t s);
 int apply_level(Plan& P, int prec, int l, const void* u, void* y, hipStream_t s);
+int apply_tail(Plan& P, int prec, const void* b, void* x, hipStream_t s);
 
 // ---- errors -----------------------------------------------------------------
 void set_error(const std::string& msg);

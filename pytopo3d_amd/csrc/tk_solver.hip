@@ -386,7 +386,7 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
   if (lt >= 1) {
     LevelBuf<T>& C = buf<T>(P.lv[lt]);
     join_coarse(P, s);   // the tail reads the coarsest inverse
-    launch_tail<T>(P, rhs[lt], C.x, tail_coefs(P, lt), s);
+    launch_tail<T>(P, rhs[lt], C.x, tail_coefs(P, lt), s, &P.sc->done);
     cur[lt] = C.x;
   } else {
     LevelBuf<T>& C = buf<T>(P.lv[nl - 1]);
@@ -821,6 +821,24 @@ int apply_level(Plan& P, int prec, int l, const void* u, void* y, hipStream_t s)
     launch_apply<double>(P, P.lv[l], P.lv[l].f64.s, (const double*)u, (double*)y, nullptr, 0, s);
   else
     launch_apply<float>(P, P.lv[l], P.lv[l].f32.s, (const float*)u, (float*)y, nullptr, 0, s);
+  TK_HIP(hipGetLastError());
+  return 0;
+}
+
+// x = the V-cycle's single-workgroup part on the tail level for right-hand side
+// b (zero on the level's fixed DOFs), with the smoothing coefficients the
+// V-cycle uses there.
+int apply_tail(Plan& P, int prec, const void* b, void* x, hipStream_t s) {
+  if (P.tail_level < 1) {
+    set_error("this plan's V-cycle has no single-workgroup tail");
+    return 1;
+  }
+  join_coarse(P, s);
+  const TailCoef cf = tail_coefs(P, P.tail_level);
+  if (prec == 0)
+    launch_tail<double>(P, (const double*)b, (double*)x, cf, s, nullptr);
+  else
+    launch_tail<float>(P, (const float*)b, (float*)x, cf, s, nullptr);
   TK_HIP(hipGetLastError());
   return 0;
 }

@@ -3,17 +3,27 @@
 // then the prolongation and the post-smoothing back on the second-coarsest
 // level.  Those two levels hold a few hundred nodes (9x5x5 and 5x3x3 at
 // 256x128x128): level by level they cost a dozen launches per V-cycle of ~5 us
-// each, nearly all launch and memory latency.  Here one workgroup keeps the
-// level state in LDS and runs all of it in one launch, separated by barriers.
+// each, nearly all launch and memory latency.  Here one workgroup of 1024
+// threads keeps the level state on chip and runs all of it in one launch,
+// separated by barriers.
 //
-// The arithmetic is that of the level-by-level kernels: K u in the
-// Walsh-Hadamard sector form of the element kernel (one thread per element
-// writes its 24 corner outputs to LDS, one thread per node sums its <= 8 in a
-// fixed order; inputs masked at fixed DOFs, identity rows there), the
-// Chebyshev steps d = c1 d + c2 D^-1 (b - K x), x' = x + d, the trilinear
-// restriction and prolongation of tk_transfer.hpp, and the stored dense
-// inverse of the coarsest operator.  (A node-gather operator with the dense
-// KE, one node per thread, took ~10 us per step on one compute unit.)
+// The work is spread so that each phase is a short chain of instructions per
+// lane (the level is far too small to be limited by throughput):
+//   - element products: 8 lanes per element, one per corner.  The
+//     Walsh-Hadamard transform of the sector form (tk_sector.hpp) runs as
+//     three butterflies across the 8 lanes (DPP lane exchanges), the sector
+//     blocks couple each lane's modes with two partner lanes' modes, and the
+//     inverse butterflies leave corner j's outputs on lane j, which stores
+//     them to LDS;
+//   - node phase: one thread per DOF sums its <= 8 element outputs in a fixed
+//     corner order (deterministic) and applies the Chebyshev step
+//     d = c1 d + c2 D^-1 (b - K x), x' = x + d, with b, D^-1, d and x held in
+//     its registers (x is also written to LDS for the next element phase);
+//   - the trilinear restriction and prolongation of tk_transfer.hpp, and the
+//     stored dense inverse of the coarsest operator, its product split into
+//     partial sums over the threads.
+// Inputs are masked at fixed DOFs and those rows are the identity, as in the
+// level-by-level kernels.
 #include "tk_plan.hpp"
 #include "tk_sector.hpp"
 #include "tk_transfer.hpp"
@@ -22,16 +32,77 @@
 
 namespace tk {
 
-constexpr int kTailThreads = 256;
+constexpr int kTailThreads = 1024;
+constexpr int kTailElPass = 2;    // element passes: 128 elements of 8 lanes each per pass
+constexpr int kTailDofPass = 2;   // DOFs owned per thread
 
-// LDS layout of the tail for level g and coarsest grid gc (element type T):
-// sector entries, x (two buffers), d, b, D^-1, element scales, the element
-// outputs (24 per element), coarse right-hand side and solution, optionally a
-// copy of the coarsest inverse (16-byte aligned), then the fixed bits of both
-// grids.  Offsets in elements of T.
+// Index of sector entry (chi, row component c, column component c') in the
+// 45 entries of tk_sector.hpp, -1 where the block is zero.
+__constant__ signed char kTailSecIdx[8][3][3] = {
+    {{0, 1, 2}, {3, 4, 5}, {6, 7, 8}},
+    {{-1, -1, -1}, {-1, 9, 10}, {-1, 11, 12}},
+    {{13, -1, 14}, {-1, -1, -1}, {15, -1, 16}},
+    {{17, 18, -1}, {19, 20, -1}, {-1, -1, 21}},
+    {{22, 23, -1}, {24, 25, -1}, {-1, -1, -1}},
+    {{26, -1, 27}, {-1, 30, -1}, {28, -1, 29}},
+    {{35, -1, -1}, {-1, 31, 32}, {-1, 33, 34}},
+    {{36, 37, 38}, {39, 40, 41}, {42, 43, 44}}};
+
+// Value of lane (l ^ M) within each group of 8 lanes, M in 1..7: a DPP quad
+// permutation for M < 4; the half-row mirror (l ^ 7) followed by the quad
+// permutation of M ^ 7 for M >= 4.  Every lane of the wave must be active.
+template <int M>
+__device__ __forceinline__ int lane_xor_i(int v) {
+  static_assert(M > 0 && M < 8, "lanes of one 8-lane group");
+  constexpr int kQuad[4] = {0xE4, 0xB1, 0x4E, 0x1B};   // quad_perm: l, l ^ 1, l ^ 2, l ^ 3
+  if constexpr (M >= 4) {
+    v = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    if constexpr ((M ^ 7) != 0) v = __builtin_amdgcn_mov_dpp(v, kQuad[M ^ 7], 0xF, 0xF, false);
+    return v;
+  } else {
+    return __builtin_amdgcn_mov_dpp(v, kQuad[M], 0xF, 0xF, false);
+  }
+}
+template <int M>
+__device__ __forceinline__ float lane_xor(float v) {
+  return __builtin_bit_cast(float, lane_xor_i<M>(__builtin_bit_cast(int, v)));
+}
+template <int M>
+__device__ __forceinline__ double lane_xor(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)lane_xor_i<M>((int)(unsigned)u);
+  const unsigned hi = (unsigned)lane_xor_i<M>((int)(unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// One butterfly of the corner transform across the lanes of an element, on
+// the corner bit M: (lo, hi) -> (lo + hi, hi - lo); the inverse
+// (sm, df) -> (sm - df, sm + df).  The same butterflies as face_fwd / face_inv
+// and the z butterfly of the element kernel.
+template <int M, typename T>
+__device__ __forceinline__ T wht_fwd(T v, bool hi) {
+  const T p = lane_xor<M>(v);
+  return hi ? v - p : v + p;
+}
+template <int M, typename T>
+__device__ __forceinline__ T wht_inv(T v, bool hi) {
+  const T p = lane_xor<M>(v);
+  return hi ? p + v : v - p;
+}
+
+// LDS layout of the tail for level g and coarsest grid gc (element type T),
+// offsets in elements of T: x, the residual, the element outputs (24 rows of
+// ne_pad, then one zero), coarse right-hand side and solution, the coarse
+// solve's partial sums (kTailThreads doubles), optionally the coarsest inverse
+// (16-byte aligned); then the fixed bits of both grids.
+__host__ __device__ inline int tail_ne_pad(const Grid& g) {
+  // a multiple of 64 plus 8: the 64 lanes of a wave (8 elements x 8 corners)
+  // write 64 different banks
+  return (g.nele + 63) / 64 * 64 + 8;
+}
 template <typename T>
 __host__ __device__ inline size_t tail_inv_offset(const Grid& g, const Grid& gc) {
-  const size_t vals = 48 + 5 * (size_t)g.ndof + g.nele + 24 * (size_t)g.nele + 2 * (size_t)gc.ndof;
+  const size_t vals = 2 * (size_t)g.ndof + 24 * (size_t)tail_ne_pad(g) + 1 + 2 * (size_t)gc.ndof +
+                      kTailThreads * sizeof(double) / sizeof(T);
   constexpr size_t a = 16 / sizeof(T);
   return (vals + a - 1) / a * a;
 }
@@ -48,206 +119,256 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     const uint8_t* __restrict__ fixed, const uint8_t* __restrict__ fixed_c,
     const T* __restrict__ dinv, const T* __restrict__ b, const T* __restrict__ inv,
     T* __restrict__ xout, TailCoef cf, int inv_lds, const int* __restrict__ done) {
-  if (*done) return;
+  if (done && *done) return;
   extern __shared__ __align__(16) unsigned char tail_mem[];
-  T* kb = reinterpret_cast<T*>(tail_mem);   // 45 sector entries (48 slots)
-  // x, two buffers; chosen by offset (a runtime-indexed pointer array loses
-  // the LDS address space and turns every access into a flat one)
-  T* X0 = kb + 48;
-  auto X = [&](int w) -> T* { return X0 + (w ? g.ndof : 0); };
-  T* D = X0 + 2 * g.ndof;
-  T* B = D + g.ndof;
-  T* DI = B + g.ndof;
-  T* S = DI + g.ndof;
-  T* EY = S + g.nele;          // element outputs: [element][corner f + 4 oz][component]
-  T* BC = EY + 24 * g.nele;
+  const int NE = tail_ne_pad(g);
+  T* X = reinterpret_cast<T*>(tail_mem);   // iterate
+  T* R = X + g.ndof;                        // residual
+  T* EY = R + g.ndof;                       // element outputs [corner j][component c][element]
+  const int zslot = 24 * NE;                // EY[zslot] = 0: the output of a missing element
+  T* BC = EY + zslot + 1;
   T* XC = BC + gc.ndof;
-  T* IV = kb + tail_inv_offset<T>(g, gc);   // the coarsest inverse, when inv_lds
+  double* PS = reinterpret_cast<double*>(XC + gc.ndof);
+  T* IV = X + tail_inv_offset<T>(g, gc);    // the coarsest inverse, when inv_lds
   uint8_t* F = reinterpret_cast<uint8_t*>(IV + (inv_lds ? gc.ndof * gc.ndof : 0));
   uint8_t* FC = F + g.nnode;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const T c0 = (T)cf.c0;
+  const int tid = threadIdx.x;
   const int n2 = gc.ndof * gc.ndof;
+  const int nxy = g.nnx * g.nny, exy = g.nelx * g.nely;
+
+  // ---- element lanes: element e = pass * 128 + tid / 8, corner j = tid % 8
+  // (j = ox + 2 oy + 4 oz, the bit order of the corner transform)
+  const int j = tid & 7;
+  const int npass = (g.nele + 127) >> 7;
+  T kc[3][3];   // this lane's sector rows: output component c, input component c'
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int c2 = 0; c2 < 3; ++c2) {
+      const int k = kTailSecIdx[j ^ (1 << c)][c][c2];
+      kc[c][c2] = k >= 0 ? sec[k] : T(0);
+    }
+  int nd[kTailElPass];       // corner node of the lane's element (-1: no element)
+  T se[kTailElPass];         // element scale
+#pragma unroll
+  for (int p = 0; p < kTailElPass; ++p) {
+    nd[p] = -1;
+    se[p] = T(0);
+    const int e = p * 128 + (tid >> 3);
+    if (p < npass && e < g.nele) {
+      const int ey = e % g.nely, t = e / g.nely, ex = t % g.nelx, ez = t / g.nelx;
+      nd[p] = (ey + ((j >> 1) & 1)) + (ex + (j & 1)) * g.nny + (ez + (j >> 2)) * nxy;
+      se[p] = sl[e];
+    }
+  }
+  // ---- owned DOFs: i = q * kTailThreads + tid; b, D^-1, d and x in registers
+  T rb[kTailDofPass], rdi[kTailDofPass], rd[kTailDofPass], rx[kTailDofPass];
+  int ebase[kTailDofPass];   // EY offset of corner 0's element for the DOF's component
+  int emask[kTailDofPass];   // bit k: the node's corner-k element exists; bit 8: DOF fixed
+#pragma unroll
+  for (int q = 0; q < kTailDofPass; ++q) {
+    const int i = q * kTailThreads + tid;
+    rb[q] = rdi[q] = rd[q] = rx[q] = T(0);
+    ebase[q] = 0;
+    emask[q] = 0;
+    if (i < g.ndof) {
+      rb[q] = b[i];
+      rdi[q] = dinv[i];
+      const int n = i / 3, c = i - 3 * n;
+      int iy, ix, iz;
+      grid_node_coords(g, n, iy, ix, iz);
+      ebase[q] = c * NE + iy + ix * g.nely + iz * exy;
+      int m = ((fixed[n] >> c) & 1) << 8;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+        if ((unsigned)(iy - oy) < (unsigned)g.nely && (unsigned)(ix - ox) < (unsigned)g.nelx &&
+            (unsigned)(iz - oz) < (unsigned)g.nelz)
+          m |= 1 << k;
+      }
+      emask[q] = m;
+    }
+  }
   if (inv_lds) {
     // 16-byte copies, several in flight per thread
     const int n16 = n2 * (int)sizeof(T) / 16;
     const uint4* __restrict__ src = reinterpret_cast<const uint4*>(inv);
     uint4* dst = reinterpret_cast<uint4*>(IV);
-#pragma unroll 8
-    for (int i = tid; i < n16; i += nt) dst[i] = src[i];
-    for (int i = n16 * (16 / (int)sizeof(T)) + tid; i < n2; i += nt) IV[i] = inv[i];
-  }
-  for (int i = tid; i < kSecN; i += nt) kb[i] = sec[i];
 #pragma unroll 4
-  for (int i = tid; i < g.ndof; i += nt) {
-    B[i] = b[i];
-    DI[i] = dinv[i];
+    for (int i = tid; i < n16; i += kTailThreads) dst[i] = src[i];
+    for (int i = n16 * (16 / (int)sizeof(T)) + tid; i < n2; i += kTailThreads) IV[i] = inv[i];
   }
-  for (int i = tid; i < g.nele; i += nt) S[i] = sl[i];
-  for (int i = tid; i < g.nnode; i += nt) F[i] = fixed[i];
-  for (int i = tid; i < gc.nnode; i += nt) FC[i] = fixed_c[i];
+  for (int i = tid; i < g.nnode; i += kTailThreads) F[i] = fixed[i];
+  for (int i = tid; i < gc.nnode; i += kTailThreads) FC[i] = fixed_c[i];
+  if (tid == 0) EY[zslot] = T(0);
   __syncthreads();
+  // fixed bits of the element lanes' corner nodes
+  int cfix[kTailElPass];
+#pragma unroll
+  for (int p = 0; p < kTailElPass; ++p) cfix[p] = nd[p] >= 0 ? F[nd[p]] : 7;
 
-  const int nxy = g.nnx * g.nny, exy = g.nelx * g.nely;
-  // EY = the element products s_e KE u_e (inputs masked at fixed DOFs), in the
-  // Walsh-Hadamard sector form of tk_sector.hpp: per component the corner
-  // values of the bottom (oz = 0) and top face, index f = ox + 2 oy, are
-  // face-transformed and combined, the sector blocks applied, and the outputs
-  // transformed back; corner (ox, oy, oz) is EY[e][f + 4 oz]
-  auto element_products = [&](const T* u) {
-    for (int e = tid; e < g.nele; e += nt) {
-      const int ey = e % g.nely, t = e / g.nely, ex = t % g.nelx, ez = t / g.nelx;
-      T Fb[3][4], Ft[3][4];
+  // EY = the element products s_e KE u_e (inputs masked at fixed DOFs) in the
+  // sector form: each lane loads its corner, three butterflies across the 8
+  // lanes give the Walsh-Hadamard modes (lane j holds mode j of each
+  // component), the sector blocks couple mode j of component c with modes
+  // j ^ e_c ^ e_c' of the others (three lane exchanges), and the inverse
+  // butterflies return corner j's outputs
+  const bool h1 = j & 1, h2 = j & 2, h4 = j & 4;
+  auto element_products = [&]() __attribute__((always_inline)) {
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const int m0 = (ey + (f >> 1)) + (ex + (f & 1)) * g.nny + ez * nxy;
-        const int m1 = m0 + nxy;
-        const uint8_t f0 = F[m0], f1 = F[m1];
+    for (int p = 0; p < kTailElPass; ++p) {
+      if (p >= npass) break;   // uniform: every lane of the wave takes part
+      T u[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          Fb[c][f] = ((f0 >> c) & 1) ? T(0) : u[3 * m0 + c];
-          Ft[c][f] = ((f1 >> c) & 1) ? T(0) : u[3 * m1 + c];
-        }
-      }
-      T a[3][8], o[3][8];
+      for (int c = 0; c < 3; ++c) u[c] = (nd[p] >= 0 && !((cfix[p] >> c) & 1)) ? X[3 * nd[p] + c] : T(0);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        face_fwd(Fb[c]);
-        face_fwd(Ft[c]);
-        a[c][0] = T(0);
+      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<1>(u[c], h1);
 #pragma unroll
-        for (int f = 1; f < 4; ++f) a[c][f] = Fb[c][f] + Ft[c][f];
+      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<2>(u[c], h2);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) a[c][f + 4] = Ft[c][f] - Fb[c][f];
-      }
-      sector_apply<T>(kb, S[e], a, o);
-      T* y = EY + 24 * e;
+      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<4>(u[c], h4);
+      const T g30 = lane_xor<3>(u[0]), g31 = lane_xor<3>(u[1]);
+      const T g50 = lane_xor<5>(u[0]), g52 = lane_xor<5>(u[2]);
+      const T g61 = lane_xor<6>(u[1]), g62 = lane_xor<6>(u[2]);
+      const T s = se[p];
+      T o[3];
+      o[0] = s * (kc[0][0] * u[0] + kc[0][1] * g31 + kc[0][2] * g52);
+      o[1] = s * (kc[1][0] * g30 + kc[1][1] * u[1] + kc[1][2] * g62);
+      o[2] = s * (kc[2][0] * g50 + kc[2][1] * g61 + kc[2][2] * u[2]);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T nb[4], nt4[4];
-        nb[0] = -o[c][4];
-        nt4[0] = o[c][4];
+      for (int c = 0; c < 3; ++c) o[c] = wht_inv<1>(o[c], h1);
 #pragma unroll
-        for (int f = 1; f < 4; ++f) {
-          nb[f] = o[c][f] - o[c][f + 4];
-          nt4[f] = o[c][f] + o[c][f + 4];
-        }
-        face_inv(nb);
-        face_inv(nt4);
+      for (int c = 0; c < 3; ++c) o[c] = wht_inv<2>(o[c], h2);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          y[3 * f + c] = nb[f];
-          y[3 * (f + 4) + c] = nt4[f];
-        }
-      }
+      for (int c = 0; c < 3; ++c) o[c] = wht_inv<4>(o[c], h4);
+      const int e = p * 128 + (tid >> 3);
+      if (nd[p] >= 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) EY[(3 * j + c) * NE + e] = o[c];
     }
     __syncthreads();
   };
-  // (K u)[node n]: the <= 8 element outputs at n, summed in a fixed order
-  auto node_sum = [&](int n, T acc[3]) {
-    int iy, ix, iz;
-    grid_node_coords(g, n, iy, ix, iz);
-    acc[0] = acc[1] = acc[2] = T(0);
+  // (K u) at owned DOF q: the <= 8 element outputs, summed in corner order
+  auto node_sum = [&](int q) __attribute__((always_inline)) -> T {
+    T acc = T(0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int oy = (k >> 1) & 1, ox = k & 1, oz = k >> 2;
-      const int ey = iy - oy, ex = ix - ox, ez = iz - oz;
-      if ((unsigned)ey >= (unsigned)g.nely || (unsigned)ex >= (unsigned)g.nelx ||
-          (unsigned)ez >= (unsigned)g.nelz)
-        continue;
-      const T* y = EY + 24 * (ey + ex * g.nely + ez * exy) + 3 * ((ox + 2 * oy) + 4 * oz);
-      acc[0] += y[0];
-      acc[1] += y[1];
-      acc[2] += y[2];
+      const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+      const int a = ebase[q] + 3 * k * NE - (oy + ox * g.nely + oz * exy);
+      acc += EY[((emask[q] >> k) & 1) ? a : zslot];
     }
+    return acc;
   };
-
-  int cur = 0;
-  // one Chebyshev step from X(cur) into X(cur ^ 1): d = c1 d + c2 D^-1 (b - K x)
-  // (first: d = c2 D^-1 (b - K x))
-  auto step = [&](bool first, T c1, T c2) {
-    const T* x = X(cur);
-    T* xo = X(cur ^ 1);
-    element_products(x);
-    for (int n = tid; n < g.nnode; n += nt) {
-      T acc[3];
-      node_sum(n, acc);
-      const uint8_t f = F[n];
+  // one Chebyshev step d = c1 d + c2 D^-1 (b - K x), x += d (first: c1 = 0)
+  auto step = [&](bool first, T c1, T c2) __attribute__((always_inline)) {
+    element_products();
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int i = 3 * n + c;
-        const T yc = ((f >> c) & 1) ? x[i] : acc[c];
-        const T dd = first ? c2 * DI[i] * (B[i] - yc) : c1 * D[i] + c2 * DI[i] * (B[i] - yc);
-        D[i] = dd;
-        xo[i] = x[i] + dd;
+    for (int q = 0; q < kTailDofPass; ++q) {
+      const int i = q * kTailThreads + tid;
+      if (i < g.ndof) {
+        const T ku = node_sum(q);
+        const T yc = (emask[q] >> 8) ? rx[q] : ku;
+        const T dd = first ? c2 * rdi[q] * (rb[q] - yc) : c1 * rd[q] + c2 * rdi[q] * (rb[q] - yc);
+        rd[q] = dd;
+        rx[q] += dd;
+        X[i] = rx[q];
       }
     }
     __syncthreads();
-    cur ^= 1;
   };
 
   // pre-smoothing from x = 0: the first step is d = c0 D^-1 b
-  for (int i = tid; i < g.ndof; i += nt) {
-    const T dd = c0 * DI[i] * B[i];
-    D[i] = dd;
-    X(cur)[i] = dd;
+  const T c0 = (T)cf.c0;
+#pragma unroll
+  for (int q = 0; q < kTailDofPass; ++q) {
+    const int i = q * kTailThreads + tid;
+    if (i < g.ndof) {
+      rd[q] = c0 * rdi[q] * rb[q];
+      rx[q] = rd[q];
+      X[i] = rx[q];
+    }
   }
   __syncthreads();
   for (int k = 1; k < cf.deg; ++k) step(false, (T)cf.c1[k], (T)cf.c2[k]);
 
-  // residual (0 on fixed DOFs) into the free buffer, restricted to the coarsest grid
-  {
-    const T* x = X(cur);
-    T* r = X(cur ^ 1);
-    element_products(x);
-    for (int n = tid; n < g.nnode; n += nt) {
-      T acc[3];
-      node_sum(n, acc);
-      const uint8_t f = F[n];
+  // residual (0 on fixed DOFs), restricted to the coarsest grid
+  element_products();
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        r[3 * n + c] = ((f >> c) & 1) ? T(0) : B[3 * n + c] - acc[c];
-    }
-    __syncthreads();
-    for (int I = tid; I < gc.nnode; I += nt) {
-      int Iy, Ix, Iz;
-      grid_node_coords(gc, I, Iy, Ix, Iz);
-      T o[3];
-      restrict_at(g, Iy, Ix, Iz, FC[I], r, o);
-      BC[3 * I] = o[0];
-      BC[3 * I + 1] = o[1];
-      BC[3 * I + 2] = o[2];
-    }
-    __syncthreads();
+  for (int q = 0; q < kTailDofPass; ++q) {
+    const int i = q * kTailThreads + tid;
+    if (i < g.ndof) R[i] = (emask[q] >> 8) ? T(0) : rb[q] - node_sum(q);
   }
-
-  // coarsest solve with the stored inverse (symmetric: row i read as column i,
-  // coalesced over the threads), accumulated in double; from the LDS copy when
-  // it fits, else from global memory with many loads in flight
-  if (inv_lds) {
-    for (int i = tid; i < gc.ndof; i += nt) {
-      double acc = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < gc.ndof; ++j) acc += (double)IV[j * gc.ndof + i] * (double)BC[j];
-      XC[i] = (T)acc;
-    }
-  } else {
-    for (int i = tid; i < gc.ndof; i += nt) {
-      double acc = 0.0;
-#pragma unroll 16
-      for (int j = 0; j < gc.ndof; ++j) acc += (double)inv[j * gc.ndof + i] * (double)BC[j];
-      XC[i] = (T)acc;
-    }
+  __syncthreads();
+  if (tid < gc.ndof) {
+    const int I = tid / 3, c = tid - 3 * I;
+    int Iy, Ix, Iz;
+    grid_node_coords(gc, I, Iy, Ix, Iz);
+    T o[3];
+    restrict_at(g, Iy, Ix, Iz, FC[I], R, o);
+    BC[tid] = o[c];
   }
   __syncthreads();
 
+  // coarsest solve with the stored inverse (symmetric: column i read as row i),
+  // split over the threads: part p of output i sums j = p, p + np, ... in
+  // double, then the np parts are added in order
+  {
+    const int n = gc.ndof, np = kTailThreads / n;
+    if (tid < np * n) {
+      const int p = tid / n, i = tid - p * n;
+      double acc = 0.0;
+      if (inv_lds) {
+#pragma unroll 4
+        for (int jj = p; jj < n; jj += np) acc += (double)IV[jj * n + i] * (double)BC[jj];
+      } else {
+#pragma unroll 8
+        for (int jj = p; jj < n; jj += np) acc += (double)inv[jj * n + i] * (double)BC[jj];
+      }
+      PS[tid] = acc;
+    }
+    __syncthreads();
+    if (tid < n) {
+      double acc = 0.0;
+      for (int p = 0; p < np; ++p) acc += PS[p * n + tid];
+      XC[tid] = (T)acc;
+    }
+    __syncthreads();
+  }
+
   // prolongation onto the free DOFs, then the post-smoothing
-  for (int n = tid; n < g.nnode; n += nt) prolong_node(gc, g, F, XC, X(cur), n);
+#pragma unroll
+  for (int q = 0; q < kTailDofPass; ++q) {
+    const int i = q * kTailThreads + tid;
+    if (i < g.ndof && !(emask[q] >> 8)) {
+      const int n = i / 3, c = i - 3 * n;
+      int iy, ix, iz;
+      grid_node_coords(g, n, iy, ix, iz);
+      const int y0 = iy >> 1, y1 = (iy + 1) >> 1, x0 = ix >> 1, x1 = (ix + 1) >> 1;
+      const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
+      const T wy[2] = {(iy & 1) ? T(0.5) : T(1), (iy & 1) ? T(0.5) : T(0)};
+      const T wx[2] = {(ix & 1) ? T(0.5) : T(1), (ix & 1) ? T(0.5) : T(0)};
+      const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
+      const int cxy = gc.nnx * gc.nny;
+      T a = T(0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+        const int I = (ky ? y1 : y0) + (kx ? x1 : x0) * gc.nny + (kz ? z1 : z0) * cxy;
+        a += wy[ky] * wx[kx] * wz[kz] * XC[3 * I + c];
+      }
+      rx[q] += a;
+      X[i] = rx[q];
+    }
+  }
   __syncthreads();
   step(true, T(0), c0);
   for (int k = 1; k < cf.deg; ++k) step(false, (T)cf.c1[k], (T)cf.c2[k]);
 
-  for (int i = tid; i < g.ndof; i += nt) xout[i] = X(cur)[i];
+#pragma unroll
+  for (int q = 0; q < kTailDofPass; ++q) {
+    const int i = q * kTailThreads + tid;
+    if (i < g.ndof) xout[i] = rx[q];
+  }
 }
 
 void tail_setup(Plan& P) {
@@ -258,7 +379,9 @@ void tail_setup(Plan& P) {
   const Grid& g = P.lv[lt].g;
   const Grid& gc = P.lv[lt + 1].g;
   const int deg = lt >= 2 ? P.opt.coarse_degree : P.opt.smooth_degree;
-  if (g.nnode > kTailMaxNodes || deg > kTailMaxDeg || gc.ndof > kCoarseMaxDof) return;
+  if (g.nnode > kTailMaxNodes || deg > kTailMaxDeg || gc.ndof > kCoarseMaxDof ||
+      g.nele > 128 * kTailElPass || g.ndof > kTailThreads * kTailDofPass || gc.ndof > kTailThreads)
+    return;
   const size_t lds64 = tail_lds<double>(g, gc, false), lds32 = tail_lds<float>(g, gc, false);
   if ((int)lds64 > P.lds_limit) return;
   // the coarsest inverse joins the LDS state where it fits
@@ -273,8 +396,9 @@ void tail_setup(Plan& P) {
   P.tail_level = lt;
 }
 
+// done == nullptr: run whatever the solver's flag says (tk_tail_apply)
 template <typename T>
-void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s) {
+void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s, const int* done) {
   const int lt = P.tail_level;
   const Level& L = P.lv[lt];
   const Level& C = P.lv[lt + 1];
@@ -295,10 +419,12 @@ void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), tail_lds<T>(L.g, C.g, inv_lds), s,
                      L.g, C.g, sec, B->s, L.fixed, C.fixed, B->dinv, b, inv, x, cf, (int)inv_lds,
-                     &P.sc->done);
+                     done);
 }
 
-template void launch_tail<float>(Plan&, const float*, float*, const TailCoef&, hipStream_t);
-template void launch_tail<double>(Plan&, const double*, double*, const TailCoef&, hipStream_t);
+template void launch_tail<float>(Plan&, const float*, float*, const TailCoef&, hipStream_t,
+                                 const int*);
+template void launch_tail<double>(Plan&, const double*, double*, const TailCoef&, hipStream_t,
+                                  const int*);
 
 }  // namespace tk

@@ -4,9 +4,11 @@ oracle evaluated on the CPU, the HIP path called through the C ABI)."""
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 import torch
 
 import oracle
+from oracle.simp import _interp_1d
 from pytopo3d_amd import _native
 from pytopo3d_amd.problem import element_flags, support_node_flags
 
@@ -45,3 +47,52 @@ def random_masks(rng, shape, p_solid=0.1, p_void=0.1):
 
 def note(*args):
     print("[case]", *args, flush=True)
+
+
+def coarsen_scales(s, nelx, nely, nelz):
+    """Element scales of the next multigrid level: a quarter of the sum of the
+    (up to) 8 children; returns them and the level's element counts."""
+    S = s.reshape((nely, nelx, nelz), order="F")
+    cx, cy, cz = (nelx + 1) // 2, (nely + 1) // 2, (nelz + 1) // 2
+    out = np.zeros((cy, cx, cz))
+    for oy in range(2):
+        for ox in range(2):
+            for oz in range(2):
+                sub = S[oy::2, ox::2, oz::2]
+                out[:sub.shape[0], :sub.shape[1], :sub.shape[2]] += sub
+    return 0.25 * out.ravel(order="F"), (cx, cy, cz)
+
+
+def assemble_level(s, nelx, nely, nelz, KE):
+    """sum_e s_e KE scattered with the reference's element DOFs (scipy CSR)."""
+    edof = oracle.element_dofs(nelx, nely, nelz) - 1
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    rows = np.repeat(edof, 24, axis=1).ravel()
+    cols = np.tile(edof, (1, 24)).ravel()
+    vals = (s[:, None] * KE.ravel()[None, :]).ravel()
+    return sp.coo_matrix((vals, (rows, cols)), shape=(ndof, ndof)).tocsr()
+
+
+def interp_level(dims_f):
+    """Trilinear interpolation from the next coarser level to level dims_f."""
+    nx, ny, nz = (d + 1 for d in dims_f)
+    Pn = sp.kron(_interp_1d(nz), sp.kron(_interp_1d(nx), _interp_1d(ny)),
+                 format="csr")
+    return sp.kron(Pn, sp.identity(3), format="csr")
+
+
+def coarse_fixed_bits(fixed, dims_f, dims_c):
+    """Node bits of the coarser level: the fine node at twice the coarse index."""
+    nxf, nyf, nzf = (d + 1 for d in dims_f)
+    nxc, nyc, nzc = (d + 1 for d in dims_c)
+    n = np.arange(nxc * nyc * nzc)
+    iy, ix, iz = n % nyc, (n // nyc) % nxc, n // (nxc * nyc)
+    fy, fx, fz = 2 * iy, 2 * ix, 2 * iz
+    inside = (fy < nyf) & (fx < nxf) & (fz < nzf)
+    fn = np.where(inside, fy + fx * nyf + fz * nxf * nyf, 0)
+    return np.where(inside, fixed[fn], 0).astype(np.uint8)
+
+
+def dof_mask(bits):
+    """Per-DOF booleans from per-node fixed bits."""
+    return np.stack([(bits >> c) & 1 for c in range(3)], axis=1).ravel().astype(bool)

@@ -15,61 +15,16 @@
 
 import numpy as np
 import pytest
-import scipy.sparse as sp
 import torch
 
-import oracle
-from oracle.simp import _interp_1d
-from gpu_helpers import dev, make_plan, note, to_dev
+from gpu_helpers import (assemble_level as _assemble, coarse_fixed_bits as _coarse_fixed,
+                         coarsen_scales as _coarsen, dev, dof_mask as _dof_mask,
+                         interp_level as _interp, make_plan, note, to_dev)
 from pytopo3d_amd import edge_load_force_vector
 from pytopo3d_amd.problem import support_node_flags
 
 pytestmark = pytest.mark.gpu
 EMIN = 1e-9
-
-
-def _coarsen(s, nelx, nely, nelz):
-    S = s.reshape((nely, nelx, nelz), order="F")
-    cx, cy, cz = (nelx + 1) // 2, (nely + 1) // 2, (nelz + 1) // 2
-    out = np.zeros((cy, cx, cz))
-    for oy in range(2):
-        for ox in range(2):
-            for oz in range(2):
-                sub = S[oy::2, ox::2, oz::2]
-                out[:sub.shape[0], :sub.shape[1], :sub.shape[2]] += sub
-    return 0.25 * out.ravel(order="F"), (cx, cy, cz)
-
-
-def _assemble(s, nelx, nely, nelz, KE):
-    edof = oracle.element_dofs(nelx, nely, nelz) - 1
-    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
-    rows = np.repeat(edof, 24, axis=1).ravel()
-    cols = np.tile(edof, (1, 24)).ravel()
-    vals = (s[:, None] * KE.ravel()[None, :]).ravel()
-    return sp.coo_matrix((vals, (rows, cols)), shape=(ndof, ndof)).tocsr()
-
-
-def _interp(dims_f):
-    nx, ny, nz = (d + 1 for d in dims_f)
-    Pn = sp.kron(_interp_1d(nz), sp.kron(_interp_1d(nx), _interp_1d(ny)),
-                 format="csr")
-    return sp.kron(Pn, sp.identity(3), format="csr")
-
-
-def _coarse_fixed(fixed, dims_f, dims_c):
-    """Node bits of the coarser level: the fine node at twice the coarse index."""
-    nxf, nyf, nzf = (d + 1 for d in dims_f)
-    nxc, nyc, nzc = (d + 1 for d in dims_c)
-    n = np.arange(nxc * nyc * nzc)
-    iy, ix, iz = n % nyc, (n // nyc) % nxc, n // (nxc * nyc)
-    fy, fx, fz = 2 * iy, 2 * ix, 2 * iz
-    inside = (fy < nyf) & (fx < nxf) & (fz < nzf)
-    fn = np.where(inside, fy + fx * nyf + fz * nxf * nyf, 0)
-    return np.where(inside, fixed[fn], 0).astype(np.uint8)
-
-
-def _dof_mask(bits):
-    return np.stack([(bits >> c) & 1 for c in range(3)], axis=1).ravel().astype(bool)
 
 
 def _check_level(P, level, A, fixed_bits, precision, seed, tol):
