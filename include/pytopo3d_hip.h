@@ -75,9 +75,9 @@ typedef struct tk_options {
                              <= 3 (default), 1 stencil gather (the only form for larger
                              radii) */
   int coarse_degree;      /* Chebyshev smoothing steps per pre/post smoothing on the third
-                             and coarser multigrid levels (default 7); smooth_degree applies
+                             and coarser multigrid levels (default 10); smooth_degree applies
                              to the two finest */
-  double coarse_cheb_lo;  /* low end of the smoothing interval on those levels (0.003):
+  double coarse_cheb_lo;  /* low end of the smoothing interval on those levels (0.0007):
                              they are latency-bound, so a longer, wider smoother costs
                              little there and brings the cycle closer to a coarse solve */
   int direction_slots;    /* mixed solves with the element kernel: FP32 search directions

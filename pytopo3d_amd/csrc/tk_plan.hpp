@@ -30,8 +30,8 @@ struct Level {
 
 struct Options {
   int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing, two finest levels
-  int coarse_degree = 7;         // ... on the third and coarser levels
-  double coarse_cheb_lo = 0.003; // cheb_lo on those levels
+  int coarse_degree = 10;        // ... on the third and coarser levels
+  double coarse_cheb_lo = 0.0007; // cheb_lo on those levels
   double cheb_lo = 0.06;         // smoothing interval [cheb_lo, cheb_hi] * lambda_max bound
   double cheb_hi = 1.0;
   int max_levels = kMaxLevels;
@@ -154,7 +154,7 @@ struct Plan {
 
 // Chebyshev coefficients of one smoothing pass on the single-workgroup tail
 // (step 0: d = c0 D^-1 r; step k >= 1: d = c1[k] d + c2[k] D^-1 r)
-constexpr int kTailMaxDeg = 8;
+constexpr int kTailMaxDeg = 12;
 constexpr int kTailMaxNodes = 512;
 struct TailCoef {
   int deg;

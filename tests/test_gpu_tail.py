@@ -8,8 +8,9 @@ and masked columns at the level's fixed DOFs; Chebyshev smoothing of the
 V-cycle's degree on [lo, 1] * lambda_bar from x = 0 (D = diag(A_l)), the
 residual restricted with the transpose of the trilinear interpolation (zero on
 the coarse fixed DOFs), the coarsest system solved exactly, the correction
-added on the free DOFs, and the post-smoothing.  Grids: 32x16x16 (tail level 2,
-coarse degree 7, the coarsest inverse in LDS for FP32), 16x8x8 (tail level 1,
+added on the free DOFs, and the post-smoothing (degrees and intervals from the
+plan's options).  Grids: 32x16x16 (tail level 2, the coarse degree, the
+coarsest inverse in LDS for FP32), 16x8x8 (tail level 1,
 fine degree 2) and 24x24x24 (216 tail elements and 1029 tail DOFs: two element
 passes and two DOFs per thread; 192 coarsest DOFs, the inverse read from
 global memory).
@@ -25,8 +26,6 @@ from pytopo3d_amd.problem import support_node_flags
 
 pytestmark = pytest.mark.gpu
 EMIN = 1e-9
-# tk_options defaults (smooth_degree, coarse_degree, cheb_lo, coarse_cheb_lo, cheb_hi)
-DEG_FINE, DEG_COARSE, LO_FINE, LO_COARSE, HI = 2, 7, 0.06, 0.003, 1.0
 
 
 def _with_identity_rows(A, fixed):
@@ -38,10 +37,10 @@ def _with_identity_rows(A, fixed):
     return A.tocsr()
 
 
-def _cheb_coefs(lam, level):
-    deg = DEG_COARSE if level >= 2 else DEG_FINE
-    lo = LO_COARSE if level >= 2 else LO_FINE
-    lmax, lmin = lam * HI, lam * lo
+def _cheb_coefs(lam, level, opt):
+    deg = opt["coarse_degree"] if level >= 2 else opt["smooth_degree"]
+    lo = opt["coarse_cheb_lo"] if level >= 2 else opt["cheb_lo"]
+    lmax, lmin = lam * opt["cheb_hi"], lam * lo
     theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
     sigma = theta / delta
     rho_old = 1.0 / sigma
@@ -54,8 +53,8 @@ def _cheb_coefs(lam, level):
     return deg, 1.0 / theta, c1, c2
 
 
-def _host_tail(A, Ac, Pm, fl, fc, b, lam, level):
-    deg, c0, c1, c2 = _cheb_coefs(lam, level)
+def _host_tail(A, Ac, Pm, fl, fc, b, lam, level, opt):
+    deg, c0, c1, c2 = _cheb_coefs(lam, level, opt)
     dinv = 1.0 / A.diagonal()
     x = c0 * dinv * b
     d = x.copy()
@@ -102,7 +101,7 @@ def test_tail_matches_host_cycle(nelx, nely, nelz, level, precision):
     Pm = interp_level(dims)
     b = np.random.default_rng(5).standard_normal(A.shape[0])
     b[fl] = 0.0
-    want = _host_tail(A, Ac, Pm, fl, fc, b, P.galerkin_info()["bound"], level)
+    want = _host_tail(A, Ac, Pm, fl, fc, b, P.galerkin_info()["bound"], level, P.options)
     dt = torch.float64 if precision == "fp64" else torch.float32
     x = torch.full((A.shape[0],), np.nan, dtype=dt, device=to_dev(b).device)
     P.tail_apply(to_dev(b, dt), x, precision=precision)
