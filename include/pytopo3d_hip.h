@@ -96,6 +96,13 @@ typedef struct tk_options {
                              iteration, and left the iteration counts of smooth SIMP designs
                              (256x128x128, 320x160x160) unchanged; the solve's tolerance is
                              the same either way */
+  int persistent_coarse;  /* 1 (default): the coarse levels between the third level and the
+                             single-workgroup tail that split into 32 bricks of <= 1024
+                             elements (32x16x16 and 16x8x8 at 256x128x128) run their whole
+                             V-cycle part in one launch per direction on the 32 compute units
+                             of one XCD, which meet at barriers between smoothing steps,
+                             instead of a ~5 us launch per step; 0: a launch per step.  The
+                             arithmetic is the same up to rounding order */
 } tk_options;
 
 /* Fill *opt with the defaults. */
@@ -206,6 +213,19 @@ int tk_tail_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream)
 /* The tail level of the plan's V-cycle (-1: none; then the coarse levels run
  * level by level down to the coarsest solve). */
 int tk_plan_tail_level(const tk_plan* plan, int* level);
+/* The levels of the persistent coarse cycle (tk_options.persistent_coarse):
+ * *first (-1: none) and *count, the levels first .. first + count - 1 just
+ * above the tail level. */
+int tk_plan_persistent_levels(const tk_plan* plan, int* first, int* count);
+/* x = the V-cycle's part from the first persistent level down and back (the
+ * persistent launches and the single-workgroup tail between them) for
+ * right-hand side b (zero on that level's fixed DOFs), with the hierarchy of
+ * the last tk_solve or tk_prepare in precision prec; b, x: that level's DOF
+ * count of values of the precision.  Fails when the plan has no persistent
+ * levels.  No counterpart in the reference (it has no multigrid); it lets
+ * tests check the persistent cycle against independently assembled level
+ * operators. */
+int tk_coarse_cycle_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream);
 /* One fine-grid Chebyshev smoothing step of the multigrid preconditioner,
  * with the element scales and inverse diagonal of the last tk_solve in the
  * same precision: d = c1 d + c2 D^-1 (b - K x); xo = x + d (identity rows on

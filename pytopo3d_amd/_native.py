@@ -49,6 +49,7 @@ class TkOptions(ctypes.Structure):
         ("coarse_cheb_lo", ctypes.c_double),
         ("direction_slots", ctypes.c_int),
         ("galerkin", ctypes.c_int),
+        ("persistent_coarse", ctypes.c_int),
     ]
 
 
@@ -69,6 +70,8 @@ _SIGS = {
     "tk_level_apply": (_I, [_P, _I, _I, _P, _P, _P]),
     "tk_tail_apply": (_I, [_P, _I, _P, _P, _P]),
     "tk_plan_tail_level": (_I, [_P, ctypes.POINTER(_I)]),
+    "tk_plan_persistent_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "tk_coarse_cycle_apply": (_I, [_P, _I, _P, _P, _P]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -233,6 +236,17 @@ class Plan:
         lv = _I()
         _check(lib().tk_plan_tail_level(self._h, ctypes.byref(lv)))
         return int(lv.value)
+
+    def persistent_levels(self):
+        """(first, count) of the persistent coarse cycle's levels (first -1: none)."""
+        f, n = _I(), _I()
+        _check(lib().tk_plan_persistent_levels(self._h, ctypes.byref(f), ctypes.byref(n)))
+        return int(f.value), int(n.value)
+
+    def coarse_cycle_apply(self, b, x, precision="fp64"):
+        """x = the V-cycle part from the first persistent level down and back, for b there."""
+        _check(lib().tk_coarse_cycle_apply(self._h, PRECISIONS[precision], _ptr(b), _ptr(x),
+                                           _stream()))
 
     def tail_apply(self, b, x, precision="fp64"):
         """x = the single-workgroup V-cycle tail for right-hand side b on the tail level."""

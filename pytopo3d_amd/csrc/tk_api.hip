@@ -52,7 +52,7 @@ static void free_plan(Plan* P) {
   f(P->r64); f(P->p64); f(P->q64); f(P->x64);
   f(P->r32); f(P->p32); f(P->q32); f(P->x32); f(P->pring); f(P->alpha_hist);
   f(P->ework0); f(P->ework1); f(P->ework2); f(P->ework3);
-  f(P->partials); f(P->sc);
+  f(P->partials); f(P->sc); f(P->pc_sync);
   f(P->gal_tab64); f(P->gal_tab32); f(P->gal_dw); f(P->gal_sh64); f(P->gal_sh32);
   if (P->sc_host) (void)hipHostFree(P->sc_host);
   for (int a = 0; a < 2; ++a)
@@ -130,6 +130,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     }
     P->opt.direction_slots = o->direction_slots;
     if (o->galerkin >= 0 && o->galerkin <= 2) P->opt.galerkin = o->galerkin;
+    if (o->persistent_coarse == 0 || o->persistent_coarse == 1)
+      P->opt.persistent_coarse = o->persistent_coarse;
   }
   if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
     delete P;
@@ -232,7 +234,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
         dmalloc(&P->ework3, g.nele);
   const long long big = g.ndof > g.nele ? g.ndof : g.nele;
   P->partials_len = 4 * (nblocks(big) + 16);
-  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->sc, 1);
+  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->sc, 1) | dmalloc(&P->pc_sync, 64);
   if (rc) {
     free_plan(P);
     return 2;
@@ -261,6 +263,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   e = e ? e : hipMemcpy(P->sten_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
   e = e ? e : hipMemcpy(P->sc, &zero, sizeof(Scalars), hipMemcpyHostToDevice);
   e = e ? e : hipMemset(P->eflags, 0, g.nele);
+  e = e ? e : hipMemset(P->pc_sync, 0, sizeof(unsigned) * 64);
   for (int l = 0; l < nl && e == hipSuccess; ++l)
     e = hipMemset(P->lv[l].fixed, 0, P->lv[l].g.nnode);
   const size_t lds = sizeof(double) * (size_t)nc * (nc + 1) / 2;
@@ -284,6 +287,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
       return 2;
     }
   }
+  pcycle_setup(*P);
   launch_filter_hs(*P, s);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -383,6 +387,7 @@ void tk_options_default(tk_options* o) {
   o->coarse_cheb_lo = d.coarse_cheb_lo;
   o->direction_slots = d.direction_slots;
   o->galerkin = d.galerkin;
+  o->persistent_coarse = d.persistent_coarse;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,
@@ -602,6 +607,33 @@ int tk_tail_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream)
     return 1;
   }
   return tk::apply_tail(P, prec, b, x, (hipStream_t)stream);
+}
+
+int tk_plan_persistent_levels(const tk_plan* plan, int* first, int* count) {
+  if (!plan || !plan->p || !first || !count) {
+    set_error("null plan or output");
+    return 1;
+  }
+  *first = plan->p->pc_first;
+  *count = plan->p->pc_nlev;
+  return 0;
+}
+
+int tk_coarse_cycle_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!b || !x) {
+    set_error("b and x are required");
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  if (P.last_penal < 0.0 || prec != P.last_prec) {
+    set_error("tk_coarse_cycle_apply needs a previous tk_solve or tk_prepare in this precision");
+    return 1;
+  }
+  return tk::apply_coarse_cycle(P, prec, b, x, (hipStream_t)stream);
 }
 
 int tk_plan_galerkin_info(const tk_plan* plan, int* level, double* bound) {

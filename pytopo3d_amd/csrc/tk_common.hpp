@@ -51,6 +51,7 @@ struct Scalars {
   // conjugate gradients
   double rz, pq, alpha, beta, rr, bb, tol2, rr0;
   int iter, done, maxit, stalled;
+  int pc_err;                // a barrier wait of the persistent coarse cycle expired
   // optimality-criteria bisection
   double lo, hi, lam, best_lam, best_err, target, V, ratio_scale, tol;
   double ka, kva, kb, kvb;   // evaluated multipliers nearest the crossing: V(ka) > target >= V(kb)

@@ -43,6 +43,7 @@ struct Options {
   int filter_kind = 0;           // 0: LDS tiles when the radius allows, 1: stencil gather
   int direction_slots = 0;       // mixed-solver direction ring (0: by memory, else 2..64)
   int galerkin = 0;              // Galerkin operator on the third level: 0 default (off), 1 on, 2 off
+  int persistent_coarse = 1;     // 1: the small levels above the tail in one launch per direction
 };
 
 struct Counters {
@@ -138,6 +139,12 @@ struct Plan {
   Counters gdelta[2][2][kChunkSizes][2];   // host counters added by one replay
   int lds_limit = 0;              // per-workgroup LDS bytes of the device
   int tail_level = -1;            // level whose V-cycle part runs in one workgroup (-1: none)
+  // persistent coarse cycle (tk_pcycle.hip): levels pc_first .. tail_level - 1
+  int pc_first = -1;
+  int pc_nlev = 0;
+  int pc_nb[kMaxLevels][3] = {};  // brick counts along y, x, z per level
+  unsigned* pc_sync = nullptr;    // barrier words (arrivals, generation), 128 B apart
+  int pc_box_max = 0, pc_epad_max = 0;   // its LDS layout
   bool tail_inv_lds64 = false;    // the tail keeps a copy of the coarsest inverse in LDS
   bool tail_inv_lds32 = false;
   // Galerkin operator of one level (tk_galerkin.hip): per element the Walsh
@@ -271,6 +278,13 @@ void launch_gal_diag(Plan& P, hipStream_t s);
 void tail_setup(Plan& P);
 template <typename T>
 void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s, const int* done);
+// persistent coarse cycle (tk_pcycle.hip); pcycle_setup picks its levels
+constexpr int kPcMaxLev = 4;
+void pcycle_setup(Plan& P);
+template <typename T>
+void launch_pcycle_down(Plan& P, const TailCoef& cf, hipStream_t s);
+template <typename T>
+void launch_pcycle_up(Plan& P, const TailCoef& cf, hipStream_t s);
 template <typename T>
 void launch_coarse_solve(Plan& P, const T* b, T* x, hipStream_t s);
 // energy
@@ -292,6 +306,7 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
 int setup_hierarchy(Plan& P, const double* rho, double penal, int prec, hipStream_t s);
 int apply_level(Plan& P, int prec, int l, const void* u, void* y, hipStream_t s);
 int apply_tail(Plan& P, int prec, const void* b, void* x, hipStream_t s);
+int apply_coarse_cycle(Plan& P, int prec, const void* b, void* x, hipStream_t s);
 
 // ---- errors -----------------------------------------------------------------
 void set_error(const std::string& msg);

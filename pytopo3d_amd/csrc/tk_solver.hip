@@ -370,9 +370,11 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
   T* alt[kMaxLevels];
   const T* rhs[kMaxLevels];
   rhs[0] = b0;
-  // from the tail level down, one workgroup runs the rest of the cycle
+  // from the tail level down, one workgroup runs the rest of the cycle; the
+  // persistent coarse cycle runs the levels above it, from pc_first
   const int lt = P.tail_level;
-  const int descend_to = lt >= 1 ? lt : nl - 1;
+  const bool pc = lt >= 1 && P.pc_nlev > 0;
+  const int descend_to = lt >= 1 ? (pc ? P.pc_first : lt) : nl - 1;
   for (int l = 0; l < descend_to; ++l) {
     LevelBuf<T>& B = buf<T>(P.lv[l]);
     T* x = smooth<T>(P, l, rhs[l], B.x, B.xa, 1, s);
@@ -384,10 +386,16 @@ static T* vcycle(Plan& P, const T* b0, hipStream_t s, int* dot_blocks) {
     rhs[l + 1] = C.b;
   }
   if (lt >= 1) {
+    const TailCoef cf = tail_coefs(P, lt);
+    if (pc) launch_pcycle_down<T>(P, cf, s);   // ends with the tail level's b
     LevelBuf<T>& C = buf<T>(P.lv[lt]);
     join_coarse(P, s);   // the tail reads the coarsest inverse
-    launch_tail<T>(P, rhs[lt], C.x, tail_coefs(P, lt), s, &P.sc->done);
+    launch_tail<T>(P, pc ? C.b : rhs[lt], C.x, cf, s, &P.sc->done);
     cur[lt] = C.x;
+    if (pc) {
+      launch_pcycle_up<T>(P, cf, s);   // the first persistent level's x
+      cur[P.pc_first] = buf<T>(P.lv[P.pc_first]).x;
+    }
   } else {
     LevelBuf<T>& C = buf<T>(P.lv[nl - 1]);
     join_coarse(P, s);
@@ -778,6 +786,28 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
   int rc = (prec == 0) ? pcg<double>(P, b, x, P.r64, P.p64, P.q64, !use_x0, rtol, maxit, jacobi, s)
                        : pcg_mixed(P, b, x, !use_x0, rtol, maxit, jacobi, s);
   if (rc) return rc;
+  if (P.sc_host->pc_err) {
+    // a barrier wait of the persistent coarse cycle expired (its workgroups did
+    // not all run on one XCD at once): the plan runs those levels a launch per
+    // step from now on; the captured chunks hold the persistent launches
+    P.pc_nlev = 0;
+    P.pc_first = -1;
+    for (auto& a : P.gexec)
+      for (auto& b2 : a)
+        for (auto& c : b2)
+          for (auto& ex : c)
+            if (ex) {
+              (void)hipGraphExecDestroy(ex);
+              ex = nullptr;
+            }
+    const int zero = 0;
+    TK_HIP(hipMemcpyAsync(&P.sc->pc_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemsetAsync(P.pc_sync, 0, sizeof(unsigned) * 64, s));
+    TK_HIP(hipStreamSynchronize(s));
+    set_error("the persistent coarse cycle timed out at a barrier; the plan now launches those "
+              "levels step by step: solve again");
+    return 3;
+  }
   const int c = (prec != 0 && P.use_el && P.pring) ? P.sc_host->iter % P.ring_m : 0;
   if (c > 0)
     hipLaunchKernelGGL(k_ring_fold_out, dim3(nblocks(n)), dim3(kBlock), 0, s, n, x, P.pring, c,
@@ -840,6 +870,42 @@ int apply_tail(Plan& P, int prec, const void* b, void* x, hipStream_t s) {
   else
     launch_tail<float>(P, (const float*)b, (float*)x, cf, s, nullptr);
   TK_HIP(hipGetLastError());
+  return 0;
+}
+
+// x = the persistent coarse cycle with the tail below it (the V-cycle's part
+// from level pc_first down and back) for right-hand side b on level pc_first.
+int apply_coarse_cycle(Plan& P, int prec, const void* b, void* x, hipStream_t s) {
+  if (P.pc_nlev < 1) {
+    set_error("this plan's V-cycle has no persistent coarse levels");
+    return 1;
+  }
+  const int l = P.pc_first, lt = P.tail_level;
+  const size_t n = (size_t)P.lv[l].g.ndof;
+  const int zero = 0;   // the kernels return at once when the solver's stop flag is set
+  TK_HIP(hipMemcpyAsync(&P.sc->done, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+  join_coarse(P, s);
+  const TailCoef cf = tail_coefs(P, lt);
+  if (prec == 0) {
+    TK_HIP(hipMemcpyAsync(P.lv[l].f64.b, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    launch_pcycle_down<double>(P, cf, s);
+    launch_tail<double>(P, P.lv[lt].f64.b, P.lv[lt].f64.x, cf, s, nullptr);
+    launch_pcycle_up<double>(P, cf, s);
+    TK_HIP(hipMemcpyAsync(x, P.lv[l].f64.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  } else {
+    TK_HIP(hipMemcpyAsync(P.lv[l].f32.b, b, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    launch_pcycle_down<float>(P, cf, s);
+    launch_tail<float>(P, P.lv[lt].f32.b, P.lv[lt].f32.x, cf, s, nullptr);
+    launch_pcycle_up<float>(P, cf, s);
+    TK_HIP(hipMemcpyAsync(x, P.lv[l].f32.x, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  TK_HIP(hipGetLastError());
+  TK_HIP(hipStreamSynchronize(s));
+  TK_HIP(hipMemcpy(P.sc_host, P.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  if (P.sc_host->pc_err) {
+    set_error("the persistent coarse cycle timed out at a barrier");
+    return 3;
+  }
   return 0;
 }
 

@@ -96,3 +96,81 @@ def coarse_fixed_bits(fixed, dims_f, dims_c):
 def dof_mask(bits):
     """Per-DOF booleans from per-node fixed bits."""
     return np.stack([(bits >> c) & 1 for c in range(3)], axis=1).ravel().astype(bool)
+
+
+def with_identity_rows(A, fixed):
+    """A with the rows and columns of the fixed DOFs replaced by identity ones
+    (the V-cycle's level operators mask inputs and keep fixed DOFs)."""
+    A = A.tolil()
+    idx = np.flatnonzero(fixed)
+    A[idx, :] = 0.0
+    A[:, idx] = 0.0
+    A[idx, idx] = 1.0
+    return A.tocsr()
+
+
+def cheb_coefs(lam, level, opt):
+    """Chebyshev degree and coefficients of one smoothing pass on a level (the
+    plan's options): first step d = c0 D^-1 r, then d = c1 d + c2 D^-1 r."""
+    deg = opt["coarse_degree"] if level >= 2 else opt["smooth_degree"]
+    lo = opt["coarse_cheb_lo"] if level >= 2 else opt["cheb_lo"]
+    lmax, lmin = lam * opt["cheb_hi"], lam * lo
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    sigma = theta / delta
+    rho_old = 1.0 / sigma
+    c1, c2 = [0.0], [0.0]
+    for _ in range(1, deg):
+        rho_new = 1.0 / (2.0 * sigma - rho_old)
+        c1.append(rho_new * rho_old)
+        c2.append(2.0 * rho_new / delta)
+        rho_old = rho_new
+    return deg, 1.0 / theta, c1, c2
+
+
+def level_stack(nelx, nely, nelz, rho, KE, first, nlev, emin=1e-9, penal=3.0):
+    """Host matrices of multigrid levels first .. nlev - 1, rediscretised like
+    the plan's hierarchy from the fine design rho: dicts of A (identity rows on
+    fixed DOFs), the fixed-DOF mask, the level number and the interpolation
+    from the next level."""
+    s = emin + (1.0 - emin) * rho ** penal
+    fixed = support_node_flags(nelx, nely, nelz)
+    dims = (nelx, nely, nelz)
+    out = []
+    for level in range(nlev):
+        if level >= first:
+            out.append({"A": with_identity_rows(assemble_level(s, *dims, KE), dof_mask(fixed)),
+                        "fixed": dof_mask(fixed), "level": level, "dims": dims,
+                        "P": interp_level(dims) if level + 1 < nlev else None})
+        if level + 1 < nlev:
+            s, d2 = coarsen_scales(s, *dims)
+            fixed = coarse_fixed_bits(fixed, dims, d2)
+            dims = d2
+    return out
+
+
+def host_cycle(lv, k, b, lam, opt):
+    """The V-cycle from entry k of level_stack's list down to its last entry
+    (the coarsest grid, solved exactly) and back: Chebyshev pre-smoothing from
+    x = 0 with D = diag(A), the residual restricted by P^T (zero on the coarse
+    fixed DOFs), the next level's cycle, the correction on the free DOFs, the
+    post-smoothing."""
+    if k == len(lv) - 1:
+        return np.linalg.solve(lv[k]["A"].toarray(), b)
+    A, fl, Pm = lv[k]["A"], lv[k]["fixed"], lv[k]["P"]
+    deg, c0, c1, c2 = cheb_coefs(lam, lv[k]["level"], opt)
+    dinv = 1.0 / A.diagonal()
+    x = c0 * dinv * b
+    d = x.copy()
+    for j in range(1, deg):
+        d = c1[j] * d + c2[j] * dinv * (b - A @ x)
+        x = x + d
+    r = np.where(fl, 0.0, b - A @ x)
+    bc = Pm.T @ r
+    bc[lv[k + 1]["fixed"]] = 0.0
+    x = x + np.where(fl, 0.0, Pm @ host_cycle(lv, k + 1, bc, lam, opt))
+    d = c0 * dinv * (b - A @ x)
+    x = x + d
+    for j in range(1, deg):
+        d = c1[j] * d + c2[j] * dinv * (b - A @ x)
+        x = x + d
+    return x
