@@ -234,7 +234,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
         dmalloc(&P->ework3, g.nele);
   const long long big = g.ndof > g.nele ? g.ndof : g.nele;
   P->partials_len = 4 * (nblocks(big) + 16);
-  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->sc, 1) | dmalloc(&P->pc_sync, 64);
+  rc |= dmalloc(&P->partials, P->partials_len) | dmalloc(&P->sc, 1) | dmalloc(&P->pc_sync, kPcSyncWords);
   if (rc) {
     free_plan(P);
     return 2;
@@ -263,7 +263,7 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
   e = e ? e : hipMemcpy(P->sten_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
   e = e ? e : hipMemcpy(P->sc, &zero, sizeof(Scalars), hipMemcpyHostToDevice);
   e = e ? e : hipMemset(P->eflags, 0, g.nele);
-  e = e ? e : hipMemset(P->pc_sync, 0, sizeof(unsigned) * 64);
+  e = e ? e : hipMemset(P->pc_sync, 0, sizeof(unsigned) * kPcSyncWords);
   for (int l = 0; l < nl && e == hipSuccess; ++l)
     e = hipMemset(P->lv[l].fixed, 0, P->lv[l].g.nnode);
   const size_t lds = sizeof(double) * (size_t)nc * (nc + 1) / 2;

@@ -10,8 +10,10 @@
 // that the hardware's XCC_ID register places on XCD 0 take part, ranked by an
 // atomic ticket in that XCD's L2 (the others return at once), each owning a
 // brick of the level's nodes and keeping its b, D^-1, d and x in registers.
-// The down and up launches draw tickets from two counters, and each resets the
-// other's once its own participants are ranked.  One smoothing step:
+// The down and up launches draw tickets from, and raise flags in, words of
+// their own; each resets the other's once its own participants are ranked
+// (the other launch runs only after this one has finished).  One smoothing
+// step:
 //   - the node values of every element touching the brick (the brick and a
 //     one-node halo) are loaded into LDS with agent-scope loads: they miss the
 //     compute unit's L1 and read the XCD's L2, where the other workgroups'
@@ -22,9 +24,11 @@
 //     x += d (fixed DOFs: identity rows, inputs masked);
 //   - the new x is stored (the level's two iterate buffers alternate, so no
 //     workgroup overwrites what another may still read), the stores are
-//     waited for, and the 32 workgroups meet at a barrier: an arrival count
-//     and a generation word in L2, the latter polled with agent-scope loads
-//     (~1 us, measured).
+//     waited for, and the 32 workgroups meet at a barrier: each stores the
+//     number of barriers it has reached into a flag word of its own (a cache
+//     line each) and one wave polls all 32 flags with agent-scope loads until
+//     every one has reached it (~1 us with the stores, measured; an arrival
+//     count that all 32 add to took ~2.3 us, its atomics serialising in L2).
 // The restriction to the next level is gathered by that level's owners after
 // a barrier, the prolongation by the finer level's owners.  Every wait is
 // bounded (~50 ms of the constant-rate clock): on expiry the launch records an
@@ -41,6 +45,7 @@ namespace tk {
 
 constexpr int kPcWG = 32;            // participating workgroups (one XCD)
 constexpr int kPcBlocks = 8 * 36;    // blocks per launch (8 XCDs): >= 32 land on XCD 0
+static_assert(kPcSyncWords >= 64 + 2 * 32 * kPcWG, "ticket and flag words");
 constexpr int kPcMaxBrickNodes = 512;
 constexpr int kPcMaxEl = 1024;       // elements touching one brick
 constexpr int kPcMaxBox = 1024;      // their nodes
@@ -75,8 +80,8 @@ struct PcArgs {
   const T* sec;                  // the 45 sector entries
   TailCoef cf;
   const int* done;
-  unsigned* sync;                // [0] arrivals, [32] generation, [48] / [56] tickets of
-  Scalars* sc;                   // the down / up launch
+  unsigned* sync;                // tickets: [16] down, [48] up launch; flags of
+  Scalars* sc;                   // workgroup w: [64 + 32 w] down, [1088 + 32 w] up
   int box_max;                   // LDS layout: node-box values (3 box_max), element
   int epad_max;                  // outputs (24 epad_max + 1), the 45 sector entries
 };
@@ -98,51 +103,56 @@ __device__ __forceinline__ void pc_store(T* p, T v) {
   *(pc_gptr<T>)p = v;
 }
 
-// Barrier of the kPcWG workgroups (see the top of the file).  Returns false
-// once any wait has expired (then every later barrier returns at once).
-__device__ bool pc_barrier(unsigned* sync, Scalars* sc) {
+// Barrier number j (from 0) of a launch among its kPcWG workgroups, rank w;
+// flags: the launch's flag words (32 apart).  Returns false once any wait has
+// expired (then every later barrier returns at once).
+__device__ bool pc_barrier(unsigned* flags, int w, unsigned j, Scalars* sc) {
   __builtin_amdgcn_s_waitcnt(0);   // this thread's stores are in L2
   __syncthreads();
   __shared__ int ok;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {   // one wave
     int good = pc_load(&sc->pc_err) == 0;
     if (good) {
-      const unsigned gen = pc_load(sync + 32);
-      const unsigned a = atomicAdd(sync, 1u);
-      if (a == kPcWG - 1) {
-        atomicExch(sync, 0u);
-        __builtin_amdgcn_s_waitcnt(0);
-        atomicAdd(sync + 32, 1u);
-      } else {
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        while (pc_load(sync + 32) == gen) {
-          if (__builtin_amdgcn_s_memrealtime() - t0 > kPcSpin) {
+      if (threadIdx.x == 0) pc_store(flags + 32 * w, j + 1);
+      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+      while (true) {
+        const unsigned f =
+            threadIdx.x < kPcWG ? pc_load(flags + 32 * threadIdx.x) : 0xffffffffu;
+        if (__all(f >= j + 1)) break;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kPcSpin) {
+          if (threadIdx.x == 0) {
             atomicExch(&sc->pc_err, 1);
             atomicExch(&sc->done, 1);
-            good = 0;
-            break;
           }
+          good = 0;
+          break;
         }
       }
     }
-    ok = good;
+    if (threadIdx.x == 0) ok = good;
   }
   __syncthreads();
   return ok;
 }
 
-// Element-output row length for nel elements: a multiple of 64 plus 8
-__host__ __device__ inline int pc_epad(int nel) { return (nel + 63) / 64 * 64 + 8; }
+// Clears the other launch's flags and ticket (rank 0, after its first barrier).
+__device__ __forceinline__ void pc_reset_other(unsigned* flags, unsigned* ticket) {
+  if (threadIdx.x < kPcWG) pc_store(flags + 32 * threadIdx.x, 0u);
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
 
-// Rank of this workgroup among the participants (tickets from sync[tk]), or
+// Rank of this workgroup among the participants (tickets from *ticket), or
 // -1: not on XCD 0, stopped solver, or a 33rd block on XCD 0.
-__device__ int pc_rank(unsigned* sync, int tk, const int* done, Scalars* sc) {
+__device__ int pc_rank(unsigned* ticket, const int* done, Scalars* sc) {
   if (pc_xcc() != 0 || *done || sc->pc_err) return -1;
   __shared__ int rank;
-  if (threadIdx.x == 0) rank = (int)atomicAdd(sync + tk, 1u);
+  if (threadIdx.x == 0) rank = (int)atomicAdd(ticket, 1u);
   __syncthreads();
   return rank < kPcWG ? rank : -1;
 }
+
+// Element-output row length for nel elements: a multiple of 64 plus 8
+__host__ __device__ inline int pc_epad(int nel) { return (nel + 63) / 64 * 64 + 8; }
 
 // Geometry of workgroup w's brick on a level: owned nodes [n0, n1) per axis
 // (y, x, z), the elements touching them (origin e0, extent ne) and their nodes
@@ -244,9 +254,13 @@ struct PcState {
 
   // EY = the element products of x (global) for the brick's elements
   __device__ __forceinline__ void products(const T* xg, T* XN, T* EY, const T* kb, int tid) const {
+    // all loads issued before any is used (unconditional, clamped index)
+    T v[LS];
+#pragma unroll
+    for (int s = 0; s < LS; ++s) v[s] = pc_load(xg + (lo[s] > 0 ? lo[s] : 0));
 #pragma unroll
     for (int s = 0; s < LS; ++s)
-      if (lo[s] > -2) XN[s * NT + tid] = lo[s] >= 0 ? pc_load(xg + lo[s]) : T(0);
+      if (lo[s] > -2) XN[s * NT + tid] = lo[s] >= 0 ? v[s] : T(0);
     if (tid == 0) EY[24 * epad] = T(0);
     __syncthreads();
     const int sy = B.nbn[0], sxy = B.nbn[0] * B.nbn[1];
@@ -375,8 +389,8 @@ __host__ __device__ inline size_t pc_lds_bytes(int box_max, int epad_max) {
 // from xin, written to xout, then the barrier.
 template <typename T, int NT>
 __device__ __forceinline__ bool pc_step(PcState<T, NT>& S, const T* xin, T* xout, T c1, T c2,
-                                        T* XN, T* EY, const T* kb, unsigned* sync, Scalars* sc,
-                                        int tid) {
+                                        T* XN, T* EY, const T* kb, unsigned* flags, int w,
+                                        unsigned& nbar, Scalars* sc, int tid) {
   S.products(xin, XN, EY, kb, tid);
 #pragma unroll
   for (int q = 0; q < PcState<T, NT>::DP; ++q) {
@@ -388,14 +402,16 @@ __device__ __forceinline__ bool pc_step(PcState<T, NT>& S, const T* xin, T* xout
     S.x[q] += dd;
     pc_store(xout + S.gi[q], S.x[q]);
   }
-  return pc_barrier(sync, sc);
+  return pc_barrier(flags, w, nbar++, sc);
 }
 
 template <typename T>
 __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_down(PcArgs<T> A) {
   constexpr int NT = pc_threads<T>();
-  const int w = pc_rank(A.sync, 48, A.done, A.sc), tid = threadIdx.x;
+  const int w = pc_rank(A.sync + 16, A.done, A.sc), tid = threadIdx.x;
   if (w < 0) return;
+  unsigned* flags = A.sync + 64;
+  unsigned nbar = 0;
   extern __shared__ __align__(16) unsigned char pc_mem[];
   T* XN = reinterpret_cast<T*>(pc_mem);
   T* EY = XN + 3 * A.box_max;
@@ -423,18 +439,18 @@ __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_down(PcArgs<T> A) {
       S.x[q] = S.d[q];
       pc_store(L.x0 + S.gi[q], S.x[q]);
     }
-    ok = pc_barrier(A.sync, A.sc);
-    if (l == 0 && w == 0 && tid == 0) atomicExch(A.sync + 56, 0u);   // the up launch's tickets
+    ok = pc_barrier(flags, w, nbar++, A.sc);
+    if (l == 0 && w == 0) pc_reset_other(A.sync + 64 + 32 * kPcWG, A.sync + 48);
     for (int k = 1; k < cf.deg && ok; ++k)
       ok = pc_step(S, (k & 1) ? L.x0 : L.x1, (k & 1) ? L.x1 : L.x0, (T)cf.c1[k], (T)cf.c2[k], XN,
-                   EY, KB, A.sync, A.sc, tid);
+                   EY, KB, flags, w, nbar, A.sc, tid);
     if (!ok) break;
     // residual of the pre-smoothed x (in x[(deg - 1) & 1]), 0 on fixed DOFs
     S.products(((cf.deg - 1) & 1) ? L.x1 : L.x0, XN, EY, KB, tid);
 #pragma unroll
     for (int q = 0; q < PcState<T, NT>::DP; ++q)
       if (S.gi[q] >= 0) pc_store(L.r + S.gi[q], S.fixed_dof(q) ? T(0) : S.b[q] - S.node_sum(q, EY));
-    ok = pc_barrier(A.sync, A.sc);
+    ok = pc_barrier(flags, w, nbar++, A.sc);
   }
   if (!ok) return;
   // the tail level's right-hand side, its DOFs spread over all threads
@@ -446,8 +462,10 @@ __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_down(PcArgs<T> A) {
 template <typename T>
 __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_up(PcArgs<T> A) {
   constexpr int NT = pc_threads<T>();
-  const int w = pc_rank(A.sync, 56, A.done, A.sc), tid = threadIdx.x;
+  const int w = pc_rank(A.sync + 48, A.done, A.sc), tid = threadIdx.x;
   if (w < 0) return;
+  unsigned* flags = A.sync + 64 + 32 * kPcWG;
+  unsigned nbar = 0;
   extern __shared__ __align__(16) unsigned char pc_mem[];
   T* XN = reinterpret_cast<T*>(pc_mem);
   T* EY = XN + 3 * A.box_max;
@@ -478,12 +496,12 @@ __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_up(PcArgs<T> A) {
       }
       pc_store(xb[pd ^ 1] + S.gi[q], S.x[q]);
     }
-    ok = pc_barrier(A.sync, A.sc);
-    if (l == A.nlev - 1 && w == 0 && tid == 0) atomicExch(A.sync + 48, 0u);   // the next down launch's
+    ok = pc_barrier(flags, w, nbar++, A.sc);
+    if (l == A.nlev - 1 && w == 0) pc_reset_other(A.sync + 64, A.sync + 16);
     int cur = pd ^ 1;
     for (int k = 0; k < cf.deg && ok; ++k) {
       const T c1 = k ? (T)cf.c1[k] : T(0), c2 = k ? (T)cf.c2[k] : c0;
-      ok = pc_step(S, xb[cur], xb[cur ^ 1], c1, c2, XN, EY, KB, A.sync, A.sc, tid);
+      ok = pc_step(S, xb[cur], xb[cur ^ 1], c1, c2, XN, EY, KB, flags, w, nbar, A.sc, tid);
       cur ^= 1;
     }
   }

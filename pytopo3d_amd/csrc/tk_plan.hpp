@@ -143,7 +143,7 @@ struct Plan {
   int pc_first = -1;
   int pc_nlev = 0;
   int pc_nb[kMaxLevels][3] = {};  // brick counts along y, x, z per level
-  unsigned* pc_sync = nullptr;    // barrier words (arrivals, generation), 128 B apart
+  unsigned* pc_sync = nullptr;    // its tickets and barrier flags (kPcSyncWords)
   int pc_box_max = 0, pc_epad_max = 0;   // its LDS layout
   bool tail_inv_lds64 = false;    // the tail keeps a copy of the coarsest inverse in LDS
   bool tail_inv_lds32 = false;
@@ -280,6 +280,7 @@ template <typename T>
 void launch_tail(Plan& P, const T* b, T* x, const TailCoef& cf, hipStream_t s, const int* done);
 // persistent coarse cycle (tk_pcycle.hip); pcycle_setup picks its levels
 constexpr int kPcMaxLev = 4;
+constexpr int kPcSyncWords = 64 + 2 * 32 * 32;   // tickets, then 2 x 32 flag lines
 void pcycle_setup(Plan& P);
 template <typename T>
 void launch_pcycle_down(Plan& P, const TailCoef& cf, hipStream_t s);

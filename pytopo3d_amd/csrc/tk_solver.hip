@@ -802,7 +802,7 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
             }
     const int zero = 0;
     TK_HIP(hipMemcpyAsync(&P.sc->pc_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
-    TK_HIP(hipMemsetAsync(P.pc_sync, 0, sizeof(unsigned) * 64, s));
+    TK_HIP(hipMemsetAsync(P.pc_sync, 0, sizeof(unsigned) * kPcSyncWords, s));
     TK_HIP(hipStreamSynchronize(s));
     set_error("the persistent coarse cycle timed out at a barrier; the plan now launches those "
               "levels step by step: solve again");
