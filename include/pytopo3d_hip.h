@@ -101,7 +101,8 @@ typedef struct tk_options {
                              elements (32x16x16 and 16x8x8 at 256x128x128) run their whole
                              V-cycle part in one launch per direction on the 32 compute units
                              of one XCD, which meet at barriers between smoothing steps,
-                             instead of a ~5 us launch per step; 0: a launch per step.  The
+                             instead of a ~5 us launch per step; -1: a launch per step; 0 keeps
+                             the default.  The
                              arithmetic is the same up to rounding order */
 } tk_options;
 

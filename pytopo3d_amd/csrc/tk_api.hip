@@ -130,8 +130,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     }
     P->opt.direction_slots = o->direction_slots;
     if (o->galerkin >= 0 && o->galerkin <= 2) P->opt.galerkin = o->galerkin;
-    if (o->persistent_coarse == 0 || o->persistent_coarse == 1)
-      P->opt.persistent_coarse = o->persistent_coarse;
+    if (o->persistent_coarse == 1 || o->persistent_coarse == -1)
+      P->opt.persistent_coarse = o->persistent_coarse == 1;
   }
   if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
     delete P;

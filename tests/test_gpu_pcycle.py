@@ -5,7 +5,7 @@
   independently assembled level matrices (gpu_helpers.host_cycle): 128x64x64
   (persistent levels 2 and 3, 32 bricks each), 64x32x32 (level 2) and the
   256x128x128 benchmark grid (levels 3 and 4), FP64 and FP32.
-* Solves with it and with a launch per step (tk_options.persistent_coarse = 0)
+* Solves with it and with a launch per step (tk_options.persistent_coarse = -1)
   agree: the same iteration counts within one and the same displacements to
   the solve's tolerance, FP64 and mixed, several solves in a row (captured
   chunks replayed), warm-started as the optimizer does.
@@ -64,9 +64,9 @@ def test_persistent_cycle_solves_agree_with_launch_per_step(precision):
     designs = [rng.uniform(0.05, 1.0, nelx * nely * nelz) for _ in range(3)]
     rtol = 1e-10 if precision == "fp64" else 1e-8
     out = {}
-    for pc in (1, 0):
+    for pc in (1, -1):
         P, _ = make_plan(nelx, nely, nelz, 1.5, options={"persistent_coarse": pc})
-        assert (P.persistent_levels()[1] > 0) == bool(pc)
+        assert (P.persistent_levels()[1] > 0) == (pc == 1)
         U = torch.zeros(P.ndof, dtype=torch.float64, device=dev())
         res = []
         for k, rho in enumerate(designs):
@@ -76,7 +76,7 @@ def test_persistent_cycle_solves_agree_with_launch_per_step(precision):
             note(f"  persistent {pc} solve {k}: {it} iterations, relres {rr:.2e}")
         out[pc] = res
         P.close()
-    for (it1, u1), (it0, u0) in zip(out[1], out[0]):
+    for (it1, u1), (it0, u0) in zip(out[1], out[-1]):
         d = float(np.abs(u1 - u0).max() / np.abs(u0).max())
         dc = abs(float(Fh @ u1) / float(Fh @ u0) - 1.0)
         note(f"  iterations {it1} / {it0}, displacement difference {d:.2e}, compliance {dc:.2e}")
