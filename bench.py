@@ -71,6 +71,11 @@ def _parse():
                     help="Chebyshev degree on the two finest levels (0: default)")
     ap.add_argument("--coarse-degree", type=int, default=0,
                     help="Chebyshev degree on the coarser levels (0: default)")
+    ap.add_argument("--mid-degree", type=int, default=0,
+                    help="Chebyshev degree on the coarse levels of more than 32768 elements "
+                         "(0: default)")
+    ap.add_argument("--mid-cheb-lo", type=float, default=0.0,
+                    help="smoothing interval low end on those levels (0: default)")
     ap.add_argument("--cheb-lo", type=float, default=0.0, help="smoothing interval low end (0: default)")
     ap.add_argument("--coarse-cheb-lo", type=float, default=0.0,
                     help="smoothing interval low end on the coarser levels (0: default)")
@@ -101,18 +106,23 @@ def _setup(args):
     solid, void, free = protected_masks(nelx, nely, nelz, None, None, None, None, F)
     KE = element_stiffness(None, 1.0)
     fixed = support_node_flags(nelx, nely, nelz, None)
-    # options left at 0 on the command line keep the library defaults (and
-    # are reported as such)
-    opts = {"operator_kind": args.operator_kind, "smooth_degree": args.smooth_degree,
-            "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo, "cheb_hi": args.cheb_hi,
-            "coarse_cheb_lo": args.coarse_cheb_lo, "galerkin": args.galerkin}
     problem = SIMPProblem(nelx, nely, nelz, 3.0, KE, F, fixed, solid, void,
-                          solver_options={k: v for k, v in opts.items() if v})
+                          solver_options=_solver_options(args))
     x0 = np.full((nely, nelx, nelz), 0.3)
     x0[solid] = 1.0
     x0[void] = 0.0
     target = 0.3 * int(free.sum())
     return problem, x0, target, DesignState, torch
+
+
+def _solver_options(args):
+    """tk_options fields set on the command line (options left at 0 keep the
+    library defaults, and are reported as such)."""
+    opts = {"operator_kind": args.operator_kind, "smooth_degree": args.smooth_degree,
+            "coarse_degree": args.coarse_degree, "cheb_lo": args.cheb_lo, "cheb_hi": args.cheb_hi,
+            "coarse_cheb_lo": args.coarse_cheb_lo, "galerkin": args.galerkin,
+            "mid_degree": args.mid_degree, "mid_cheb_lo": args.mid_cheb_lo}
+    return {k: v for k, v in opts.items() if v}
 
 
 def _run_path(problem, x0, target, DesignState, torch, precision, args):
@@ -365,7 +375,7 @@ def _config4(args):
     F = edge_load_force_vector(nelx, nely, nelz)
     kw = dict(volfrac=0.2, penal=3.0, rmin=4.0, disp_thres=0.5, force_vector=F, maxloop=40,
               tolx=1e-12, beta_schedule=(1.0, 2.0, 4.0, 8.0), projection_eta=0.5, move=0.2,
-              solver_rtol=args.rtol)
+              solver_rtol=args.rtol, solver_options=_solver_options(args))
     out = {"config": "320x160x160, volfrac 0.2, rmin 4, beta 1,2,4,8 x 40 iterations"}
     for prec in ("mixed", "fp64"):
         torch.cuda.synchronize()
@@ -461,6 +471,7 @@ def main():
             "smooth_degree": plan.options["smooth_degree"],
             "coarse_degree": plan.options["coarse_degree"], "cheb_lo": plan.options["cheb_lo"],
             "coarse_cheb_lo": plan.options["coarse_cheb_lo"],
+            "mid_degree": plan.options["mid_degree"], "mid_cheb_lo": plan.options["mid_cheb_lo"],
         },
         "solve_iters_mean": float(np.mean([r["solve_iters"] for r in recs_m])),
         "oc_steps_mean": float(np.mean([r["oc_steps"] for r in recs_m])),

@@ -75,9 +75,11 @@ typedef struct tk_options {
                              <= 3 (default), 1 stencil gather (the only form for larger
                              radii) */
   int coarse_degree;      /* Chebyshev smoothing steps per pre/post smoothing on the third
-                             and coarser multigrid levels (default 10); smooth_degree applies
-                             to the two finest */
-  double coarse_cheb_lo;  /* low end of the smoothing interval on those levels (0.0007):
+                             and coarser multigrid levels of at most 32768 elements (default
+                             12; those the persistent coarse cycle and the single-workgroup
+                             tail run); smooth_degree applies to the two finest, mid_degree
+                             to the coarse levels of more elements */
+  double coarse_cheb_lo;  /* low end of the smoothing interval on those levels (0.0005):
                              they are latency-bound, so a longer, wider smoother costs
                              little there and brings the cycle closer to a coarse solve */
   int direction_slots;    /* mixed solves with the element kernel: FP32 search directions
@@ -102,8 +104,11 @@ typedef struct tk_options {
                              V-cycle part in one launch per direction on the 32 compute units
                              of one XCD, which meet at barriers between smoothing steps,
                              instead of a ~5 us launch per step; -1: a launch per step; 0 keeps
-                             the default.  The
-                             arithmetic is the same up to rounding order */
+                             the default.  The arithmetic is the same up to rounding order */
+  int mid_degree;         /* Chebyshev smoothing steps on the third and coarser levels of
+                             more than 32768 elements (default 8; 64x32x32 at 256x128x128),
+                             each a launch per step */
+  double mid_cheb_lo;     /* low end of the smoothing interval on those levels (0.002) */
 } tk_options;
 
 /* Fill *opt with the defaults. */

@@ -50,6 +50,8 @@ class TkOptions(ctypes.Structure):
         ("direction_slots", ctypes.c_int),
         ("galerkin", ctypes.c_int),
         ("persistent_coarse", ctypes.c_int),
+        ("mid_degree", ctypes.c_int),
+        ("mid_cheb_lo", ctypes.c_double),
     ]
 
 

@@ -114,6 +114,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->smooth_degree > 0) P->opt.smooth_degree = o->smooth_degree;
     if (o->coarse_degree > 0) P->opt.coarse_degree = o->coarse_degree;
     if (o->coarse_cheb_lo > 0.0) P->opt.coarse_cheb_lo = o->coarse_cheb_lo;
+    if (o->mid_degree > 0) P->opt.mid_degree = o->mid_degree;
+    if (o->mid_cheb_lo > 0.0) P->opt.mid_cheb_lo = o->mid_cheb_lo;
     if (o->cheb_lo > 0.0) P->opt.cheb_lo = o->cheb_lo;
     if (o->cheb_hi > 0.0) P->opt.cheb_hi = o->cheb_hi;
     if (o->max_levels > 0) P->opt.max_levels = o->max_levels < kMaxLevels ? o->max_levels : kMaxLevels;
@@ -133,9 +135,10 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     if (o->persistent_coarse == 1 || o->persistent_coarse == -1)
       P->opt.persistent_coarse = o->persistent_coarse == 1;
   }
-  if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi)) {
+  if (!(P->opt.cheb_lo < P->opt.cheb_hi) || !(P->opt.coarse_cheb_lo < P->opt.cheb_hi) ||
+      !(P->opt.mid_cheb_lo < P->opt.cheb_hi)) {
     delete P;
-    set_error("cheb_lo and coarse_cheb_lo must be below cheb_hi");
+    set_error("cheb_lo, coarse_cheb_lo and mid_cheb_lo must be below cheb_hi");
     return 1;
   }
   {
@@ -388,6 +391,8 @@ void tk_options_default(tk_options* o) {
   o->direction_slots = d.direction_slots;
   o->galerkin = d.galerkin;
   o->persistent_coarse = d.persistent_coarse;
+  o->mid_degree = d.mid_degree;
+  o->mid_cheb_lo = d.mid_cheb_lo;
 }
 
 int tk_plan_create(tk_plan** out, int nelx, int nely, int nelz, const double* KE, double rmin,

@@ -551,12 +551,14 @@ void pcycle_setup(Plan& P) {
   P.pc_nlev = 0;
   const int lt = P.tail_level;
   if (!P.opt.persistent_coarse || lt < 3 || !P.sector_ok || !P.use_el) return;
-  // levels above the tail, coarse ones (the same smoothing coefficients as the
-  // tail), not the Galerkin level, and small enough for 32 bricks
+  // levels above the tail, coarse ones of at most kMidMinEl elements (the same
+  // smoothing coefficients as the tail), not the Galerkin level, and small
+  // enough for 32 bricks
   int first = lt, box_max = 0, epad_max = 0;
   for (int l = lt - 1; l >= 2 && lt - l <= kPcMaxLev; --l) {
     int bm = box_max, em = epad_max;
-    if (l == P.gal_level || !pc_split(P.lv[l].g, P.pc_nb[l], bm, em) ||
+    if (l == P.gal_level || P.lv[l].g.nele > kMidMinEl ||
+        !pc_split(P.lv[l].g, P.pc_nb[l], bm, em) ||
         pc_lds_bytes<double>(bm, em) > (size_t)P.lds_limit)
       break;
     box_max = bm;

@@ -28,10 +28,18 @@ struct Level {
   LevelBuf<float> f32;
 };
 
+// Coarse levels (the third and coarser) with more elements than this are
+// smoothed with Options::mid_degree / mid_cheb_lo, the smaller ones (those the
+// persistent coarse cycle and the tail can hold) with coarse_degree /
+// coarse_cheb_lo.
+constexpr int kMidMinEl = 32768;
+
 struct Options {
   int smooth_degree = 2;         // Chebyshev degree per pre/post smoothing, two finest levels
-  int coarse_degree = 10;        // ... on the third and coarser levels
-  double coarse_cheb_lo = 0.0007; // cheb_lo on those levels
+  int coarse_degree = 12;        // ... on the coarse levels of <= kMidMinEl elements
+  double coarse_cheb_lo = 0.0005; // cheb_lo on those levels
+  int mid_degree = 8;            // ... on the coarse levels of more elements
+  double mid_cheb_lo = 0.002;    // cheb_lo on those levels
   double cheb_lo = 0.06;         // smoothing interval [cheb_lo, cheb_hi] * lambda_max bound
   double cheb_hi = 1.0;
   int max_levels = kMaxLevels;

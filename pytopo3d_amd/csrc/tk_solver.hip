@@ -271,6 +271,24 @@ static int read_scalars(Plan& P, hipStream_t s) {
   return 0;
 }
 
+// Chebyshev degree and interval low end (fraction of the lambda_max bound) of
+// level l.  The coarse levels are latency-bound: a longer smoother over a
+// wider interval costs little there and strengthens the cycle; the smallest
+// ones (a few microseconds per step in the persistent cycle and the tail) take
+// the longest.
+static void level_cheb(const Plan& P, int l, int& deg, double& lo) {
+  if (l < 2) {
+    deg = P.opt.smooth_degree;
+    lo = P.opt.cheb_lo;
+  } else if (P.lv[l].g.nele > kMidMinEl) {
+    deg = P.opt.mid_degree;
+    lo = P.opt.mid_cheb_lo;
+  } else {
+    deg = P.opt.coarse_degree;
+    lo = P.opt.coarse_cheb_lo;
+  }
+}
+
 // Chebyshev smoothing of level l: `deg` steps from x (x_zero: from 0).
 // Returns the buffer holding the result (x or its ping-pong partner).
 constexpr int kProlongFuseMaxNodes = 100000;
@@ -283,15 +301,14 @@ template <typename T>
 static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_t s,
                  const T* xc = nullptr, int* dot_blocks = nullptr) {
   Level& L = P.lv[l];
-  // the coarse levels are latency-bound: a longer smoother over a wider
-  // interval costs little there and strengthens the cycle
-  const bool coarse = l >= 2;
-  const int deg = coarse ? P.opt.coarse_degree : P.opt.smooth_degree;
+  int deg;
+  double lo;
+  level_cheb(P, l, deg, lo);
   LevelBuf<T>& B = buf<T>(L);
   // the Galerkin level has its own bound of lambda_max(D^-1 A)
   const double lam = l == P.gal_level ? P.lam_gal : P.lam_bound;
   const double lmax = lam * P.opt.cheb_hi;
-  const double lmin = lam * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
+  const double lmin = lam * lo;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
@@ -333,11 +350,11 @@ static T* smooth(Plan& P, int l, const T* b, T* x, T* xa, int x_zero, hipStream_
 
 // The coefficients smooth() uses on level l (same recurrence), for the tail.
 static TailCoef tail_coefs(const Plan& P, int l) {
-  const bool coarse = l >= 2;
   TailCoef cf{};
-  cf.deg = coarse ? P.opt.coarse_degree : P.opt.smooth_degree;
+  double lo;
+  level_cheb(P, l, cf.deg, lo);
   const double lmax = P.lam_bound * P.opt.cheb_hi;
-  const double lmin = P.lam_bound * (coarse ? P.opt.coarse_cheb_lo : P.opt.cheb_lo);
+  const double lmin = P.lam_bound * lo;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;

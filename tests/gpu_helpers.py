@@ -109,11 +109,19 @@ def with_identity_rows(A, fixed):
     return A.tocsr()
 
 
-def cheb_coefs(lam, level, opt):
-    """Chebyshev degree and coefficients of one smoothing pass on a level (the
-    plan's options): first step d = c0 D^-1 r, then d = c1 d + c2 D^-1 r."""
-    deg = opt["coarse_degree"] if level >= 2 else opt["smooth_degree"]
-    lo = opt["coarse_cheb_lo"] if level >= 2 else opt["cheb_lo"]
+MID_MIN_EL = 32768   # coarse levels of more elements take mid_degree / mid_cheb_lo
+
+
+def cheb_coefs(lam, level, opt, nele):
+    """Chebyshev degree and coefficients of one smoothing pass on a level of
+    nele elements (the plan's options): first step d = c0 D^-1 r, then
+    d = c1 d + c2 D^-1 r."""
+    if level < 2:
+        deg, lo = opt["smooth_degree"], opt["cheb_lo"]
+    elif nele > MID_MIN_EL:
+        deg, lo = opt["mid_degree"], opt["mid_cheb_lo"]
+    else:
+        deg, lo = opt["coarse_degree"], opt["coarse_cheb_lo"]
     lmax, lmin = lam * opt["cheb_hi"], lam * lo
     theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
     sigma = theta / delta
@@ -157,7 +165,7 @@ def host_cycle(lv, k, b, lam, opt):
     if k == len(lv) - 1:
         return np.linalg.solve(lv[k]["A"].toarray(), b)
     A, fl, Pm = lv[k]["A"], lv[k]["fixed"], lv[k]["P"]
-    deg, c0, c1, c2 = cheb_coefs(lam, lv[k]["level"], opt)
+    deg, c0, c1, c2 = cheb_coefs(lam, lv[k]["level"], opt, int(np.prod(lv[k]["dims"])))
     dinv = 1.0 / A.diagonal()
     x = c0 * dinv * b
     d = x.copy()

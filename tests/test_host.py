@@ -125,7 +125,7 @@ def test_header_is_plain_c_and_links_from_c(tmp_path):
                     "-L", libdir, "-lpytopo3d_hip", "-Wl,-rpath," + libdir],
                    check=True, capture_output=True, text=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    assert out[:3] == ["2", "10", "1"]
+    assert out[:3] == ["2", "12", "1"]
     assert "out must not be NULL" in " ".join(out[3:])
 
 
