@@ -223,6 +223,11 @@ int tk_plan_tail_level(const tk_plan* plan, int* level);
  * *first (-1: none) and *count, the levels first .. first + count - 1 just
  * above the tail level. */
 int tk_plan_persistent_levels(const tk_plan* plan, int* first, int* count);
+/* How long a workgroup of the persistent coarse cycle waits at a barrier for
+ * the others (default 0.05 s; [0, 10]).  When a wait expires the solve returns
+ * 3 and the plan runs those levels a launch per step from then on.  No
+ * counterpart in the reference; it lets tests force that fallback. */
+int tk_plan_set_barrier_timeout(tk_plan* plan, double seconds);
 /* x = the V-cycle's part from the first persistent level down and back (the
  * persistent launches and the single-workgroup tail between them) for
  * right-hand side b (zero on that level's fixed DOFs), with the hierarchy of

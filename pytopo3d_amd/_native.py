@@ -74,6 +74,7 @@ _SIGS = {
     "tk_plan_tail_level": (_I, [_P, ctypes.POINTER(_I)]),
     "tk_plan_persistent_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "tk_coarse_cycle_apply": (_I, [_P, _I, _P, _P, _P]),
+    "tk_plan_set_barrier_timeout": (_I, [_P, _D]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
     "tk_filter_transpose2": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -244,6 +245,10 @@ class Plan:
         f, n = _I(), _I()
         _check(lib().tk_plan_persistent_levels(self._h, ctypes.byref(f), ctypes.byref(n)))
         return int(f.value), int(n.value)
+
+    def set_barrier_timeout(self, seconds):
+        """Wait limit of the persistent coarse cycle's barriers (default 0.05 s)."""
+        _check(lib().tk_plan_set_barrier_timeout(self._h, float(seconds)))
 
     def coarse_cycle_apply(self, b, x, precision="fp64"):
         """x = the V-cycle part from the first persistent level down and back, for b there."""

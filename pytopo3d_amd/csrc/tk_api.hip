@@ -624,6 +624,20 @@ int tk_plan_persistent_levels(const tk_plan* plan, int* first, int* count) {
   return 0;
 }
 
+int tk_plan_set_barrier_timeout(tk_plan* plan, double seconds) {
+  if (!plan || !plan->p) {
+    set_error("null plan");
+    return 1;
+  }
+  if (!(seconds >= 0.0 && seconds <= 10.0)) {
+    set_error("the barrier timeout must be in [0, 10] seconds");
+    return 1;
+  }
+  // the kernels read the realtime clock, which counts at 100 MHz
+  plan->p->pc_spin = (unsigned long long)(seconds * 1e8);
+  return 0;
+}
+
 int tk_coarse_cycle_apply(tk_plan* plan, int prec, const void* b, void* x, void* stream) {
   TK_PLAN_OR_FAIL(plan);
   if (!b || !x) {

@@ -49,7 +49,6 @@ static_assert(kPcSyncWords >= 64 + 2 * 32 * kPcWG, "ticket and flag words");
 constexpr int kPcMaxBrickNodes = 512;
 constexpr int kPcMaxEl = 1024;       // elements touching one brick
 constexpr int kPcMaxBox = 1024;      // their nodes
-constexpr unsigned long long kPcSpin = 5000000ull;   // 50 ms at the 100 MHz realtime clock
 
 template <typename T>
 constexpr int pc_threads() {
@@ -84,6 +83,7 @@ struct PcArgs {
   Scalars* sc;                   // workgroup w: [64 + 32 w] down, [1088 + 32 w] up
   int box_max;                   // LDS layout: node-box values (3 box_max), element
   int epad_max;                  // outputs (24 epad_max + 1), the 45 sector entries
+  unsigned long long spin;       // barrier wait limit, ticks of the 100 MHz realtime clock
 };
 
 // Device-memory accesses of data other workgroups write during the launch: the
@@ -106,7 +106,8 @@ __device__ __forceinline__ void pc_store(T* p, T v) {
 // Barrier number j (from 0) of a launch among its kPcWG workgroups, rank w;
 // flags: the launch's flag words (32 apart).  Returns false once any wait has
 // expired (then every later barrier returns at once).
-__device__ bool pc_barrier(unsigned* flags, int w, unsigned j, Scalars* sc) {
+__device__ bool pc_barrier(unsigned* flags, int w, unsigned j, Scalars* sc,
+                           unsigned long long spin) {
   __builtin_amdgcn_s_waitcnt(0);   // this thread's stores are in L2
   __syncthreads();
   __shared__ int ok;
@@ -119,7 +120,7 @@ __device__ bool pc_barrier(unsigned* flags, int w, unsigned j, Scalars* sc) {
         const unsigned f =
             threadIdx.x < kPcWG ? pc_load(flags + 32 * threadIdx.x) : 0xffffffffu;
         if (__all(f >= j + 1)) break;
-        if (__builtin_amdgcn_s_memrealtime() - t0 > kPcSpin) {
+        if (__builtin_amdgcn_s_memrealtime() - t0 > spin) {
           if (threadIdx.x == 0) {
             atomicExch(&sc->pc_err, 1);
             atomicExch(&sc->done, 1);
@@ -390,7 +391,8 @@ __host__ __device__ inline size_t pc_lds_bytes(int box_max, int epad_max) {
 template <typename T, int NT>
 __device__ __forceinline__ bool pc_step(PcState<T, NT>& S, const T* xin, T* xout, T c1, T c2,
                                         T* XN, T* EY, const T* kb, unsigned* flags, int w,
-                                        unsigned& nbar, Scalars* sc, int tid) {
+                                        unsigned& nbar, Scalars* sc, unsigned long long spin,
+                                        int tid) {
   S.products(xin, XN, EY, kb, tid);
 #pragma unroll
   for (int q = 0; q < PcState<T, NT>::DP; ++q) {
@@ -402,7 +404,7 @@ __device__ __forceinline__ bool pc_step(PcState<T, NT>& S, const T* xin, T* xout
     S.x[q] += dd;
     pc_store(xout + S.gi[q], S.x[q]);
   }
-  return pc_barrier(flags, w, nbar++, sc);
+  return pc_barrier(flags, w, nbar++, sc, spin);
 }
 
 template <typename T>
@@ -439,18 +441,18 @@ __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_down(PcArgs<T> A) {
       S.x[q] = S.d[q];
       pc_store(L.x0 + S.gi[q], S.x[q]);
     }
-    ok = pc_barrier(flags, w, nbar++, A.sc);
+    ok = pc_barrier(flags, w, nbar++, A.sc, A.spin);
     if (l == 0 && w == 0) pc_reset_other(A.sync + 64 + 32 * kPcWG, A.sync + 48);
     for (int k = 1; k < cf.deg && ok; ++k)
       ok = pc_step(S, (k & 1) ? L.x0 : L.x1, (k & 1) ? L.x1 : L.x0, (T)cf.c1[k], (T)cf.c2[k], XN,
-                   EY, KB, flags, w, nbar, A.sc, tid);
+                   EY, KB, flags, w, nbar, A.sc, A.spin, tid);
     if (!ok) break;
     // residual of the pre-smoothed x (in x[(deg - 1) & 1]), 0 on fixed DOFs
     S.products(((cf.deg - 1) & 1) ? L.x1 : L.x0, XN, EY, KB, tid);
 #pragma unroll
     for (int q = 0; q < PcState<T, NT>::DP; ++q)
       if (S.gi[q] >= 0) pc_store(L.r + S.gi[q], S.fixed_dof(q) ? T(0) : S.b[q] - S.node_sum(q, EY));
-    ok = pc_barrier(flags, w, nbar++, A.sc);
+    ok = pc_barrier(flags, w, nbar++, A.sc, A.spin);
   }
   if (!ok) return;
   // the tail level's right-hand side, its DOFs spread over all threads
@@ -496,12 +498,13 @@ __global__ __launch_bounds__(pc_threads<T>()) void k_pcycle_up(PcArgs<T> A) {
       }
       pc_store(xb[pd ^ 1] + S.gi[q], S.x[q]);
     }
-    ok = pc_barrier(flags, w, nbar++, A.sc);
+    ok = pc_barrier(flags, w, nbar++, A.sc, A.spin);
     if (l == A.nlev - 1 && w == 0) pc_reset_other(A.sync + 64, A.sync + 16);
     int cur = pd ^ 1;
     for (int k = 0; k < cf.deg && ok; ++k) {
       const T c1 = k ? (T)cf.c1[k] : T(0), c2 = k ? (T)cf.c2[k] : c0;
-      ok = pc_step(S, xb[cur], xb[cur ^ 1], c1, c2, XN, EY, KB, flags, w, nbar, A.sc, tid);
+      ok = pc_step(S, xb[cur], xb[cur ^ 1], c1, c2, XN, EY, KB, flags, w, nbar, A.sc, A.spin,
+                   tid);
       cur ^= 1;
     }
   }
@@ -631,6 +634,7 @@ static PcArgs<T> pc_args(Plan& P, const TailCoef& cf) {
   A.sc = P.sc;
   A.box_max = P.pc_box_max;
   A.epad_max = P.pc_epad_max;
+  A.spin = P.pc_spin;
   return A;
 }
 

@@ -153,6 +153,7 @@ struct Plan {
   int pc_nb[kMaxLevels][3] = {};  // brick counts along y, x, z per level
   unsigned* pc_sync = nullptr;    // its tickets and barrier flags (kPcSyncWords)
   int pc_box_max = 0, pc_epad_max = 0;   // its LDS layout
+  unsigned long long pc_spin = 5000000;  // its barrier wait limit (100 MHz clock ticks: 50 ms)
   bool tail_inv_lds64 = false;    // the tail keeps a copy of the coarsest inverse in LDS
   bool tail_inv_lds32 = false;
   // Galerkin operator of one level (tk_galerkin.hip): per element the Walsh

@@ -775,6 +775,28 @@ static int ensure_ring(Plan& P) {
   return 0;
 }
 
+// After an expired barrier wait of the persistent coarse cycle (its workgroups
+// did not all run on one XCD at once): the plan runs those levels a launch per
+// step from now on.  The captured chunks hold the persistent launches, so they
+// are dropped; the error flag and the barrier words are cleared.
+static int drop_pcycle(Plan& P, hipStream_t s) {
+  P.pc_nlev = 0;
+  P.pc_first = -1;
+  for (auto& a : P.gexec)
+    for (auto& b2 : a)
+      for (auto& c : b2)
+        for (auto& ex : c)
+          if (ex) {
+            (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
+          }
+  const int zero = 0;
+  TK_HIP(hipMemcpyAsync(&P.sc->pc_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+  TK_HIP(hipMemsetAsync(P.pc_sync, 0, sizeof(unsigned) * kPcSyncWords, s));
+  TK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 // ------------------------------------------------------------------ driver
 // flags: bit0 = U holds the initial guess, bit1 = Jacobi instead of multigrid,
 //        bit2 = skip hierarchy setup (reuse the last one for the same rho).
@@ -804,23 +826,7 @@ int solve(Plan& P, int prec, const double* rho, double penal, const double* F, d
                        : pcg_mixed(P, b, x, !use_x0, rtol, maxit, jacobi, s);
   if (rc) return rc;
   if (P.sc_host->pc_err) {
-    // a barrier wait of the persistent coarse cycle expired (its workgroups did
-    // not all run on one XCD at once): the plan runs those levels a launch per
-    // step from now on; the captured chunks hold the persistent launches
-    P.pc_nlev = 0;
-    P.pc_first = -1;
-    for (auto& a : P.gexec)
-      for (auto& b2 : a)
-        for (auto& c : b2)
-          for (auto& ex : c)
-            if (ex) {
-              (void)hipGraphExecDestroy(ex);
-              ex = nullptr;
-            }
-    const int zero = 0;
-    TK_HIP(hipMemcpyAsync(&P.sc->pc_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
-    TK_HIP(hipMemsetAsync(P.pc_sync, 0, sizeof(unsigned) * kPcSyncWords, s));
-    TK_HIP(hipStreamSynchronize(s));
+    if (int e = drop_pcycle(P, s)) return e;
     set_error("the persistent coarse cycle timed out at a barrier; the plan now launches those "
               "levels step by step: solve again");
     return 3;
@@ -920,7 +926,9 @@ int apply_coarse_cycle(Plan& P, int prec, const void* b, void* x, hipStream_t s)
   TK_HIP(hipStreamSynchronize(s));
   TK_HIP(hipMemcpy(P.sc_host, P.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
   if (P.sc_host->pc_err) {
-    set_error("the persistent coarse cycle timed out at a barrier");
+    if (int e = drop_pcycle(P, s)) return e;
+    set_error("the persistent coarse cycle timed out at a barrier; the plan now launches those "
+              "levels step by step");
     return 3;
   }
   return 0;

@@ -83,3 +83,38 @@ def test_persistent_cycle_solves_agree_with_launch_per_step(precision):
         assert abs(it1 - it0) <= 1
         assert d <= (1e-8 if precision == "fp64" else 1e-6)
         assert dc <= (1e-10 if precision == "fp64" else 1e-8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_expired_barrier_wait_falls_back_to_a_launch_per_step(precision):
+    """A barrier wait limit of zero (tk_plan_set_barrier_timeout) makes the
+    persistent cycle's own bounded waits expire at once, so every workgroup
+    still exits: the solve fails with the timeout message, the plan drops its
+    persistent levels, and the next solve runs them a launch per step, as a
+    plan made that way does."""
+    nelx, nely, nelz = 128, 64, 64
+    note("expired barrier wait", nelx, nely, nelz, precision)
+    F = to_dev(edge_load_force_vector(nelx, nely, nelz))
+    rho = to_dev(np.random.default_rng(9).uniform(0.05, 1.0, nelx * nely * nelz))
+    rtol = 1e-10 if precision == "fp64" else 1e-8
+    P, _ = make_plan(nelx, nely, nelz, 1.5)
+    assert P.persistent_levels()[1] > 0
+    P.set_barrier_timeout(0.0)
+    U = torch.zeros(P.ndof, dtype=torch.float64, device=dev())
+    with pytest.raises(RuntimeError, match="timed out"):
+        P.solve(rho, 3.0, F, U, rtol=rtol, precision=precision)
+    assert P.persistent_levels() == (-1, 0)
+    U.zero_()
+    it, rr = P.solve(rho, 3.0, F, U, rtol=rtol, precision=precision)
+    note(f"  after the fallback: {it} iterations, relres {rr:.2e}")
+    Q, _ = make_plan(nelx, nely, nelz, 1.5, options={"persistent_coarse": -1})
+    V = torch.zeros(Q.ndof, dtype=torch.float64, device=dev())
+    it0, _ = Q.solve(rho, 3.0, F, V, rtol=rtol, precision=precision)
+    assert it == it0
+    d = float((U - V).abs().max() / V.abs().max())
+    note(f"  against a launch-per-step plan: {it0} iterations, displacement difference {d:.2e}")
+    assert d <= 1e-14
+    with pytest.raises(ValueError):
+        P.set_barrier_timeout(-1.0)
+    P.close()
+    Q.close()
