@@ -1,6 +1,6 @@
 // Walsh-Hadamard sector form of the H8 element operator, shared by the
-// streaming element kernel (tk_operator2.hip) and the single-workgroup coarse
-// V-cycle tail (tk_tail.hip).  The 8 corner values of each displacement
+// streaming element kernel (tk_operator2.hip), the single-workgroup coarse
+// V-cycle tail (tk_tail.hip) and the persistent coarse cycle (tk_pcycle.hip).  The 8 corner values of each displacement
 // component are transformed per 2x2 face (index f = ox + 2 oy) and then
 // between the bottom (z = 0) and top (z = 1) faces; in that basis KE splits
 // into the 8 parity sectors described below, scaled by 1/64 on the host.
@@ -111,6 +111,104 @@ __device__ __forceinline__ void sector_apply(const T* __restrict__ k, T s, const
     o[1][5] = s * (b[3] * x0 + b[4] * x1 + b[5] * x2);
     o[2][3] = s * (b[6] * x0 + b[7] * x1 + b[8] * x2);
   }
+}
+
+// Index of sector entry (chi, row component c, column component c') in the
+// 45 entries of tk_sector.hpp, -1 where the block is zero.
+static __constant__ signed char kSecIdx[8][3][3] = {
+    {{0, 1, 2}, {3, 4, 5}, {6, 7, 8}},
+    {{-1, -1, -1}, {-1, 9, 10}, {-1, 11, 12}},
+    {{13, -1, 14}, {-1, -1, -1}, {15, -1, 16}},
+    {{17, 18, -1}, {19, 20, -1}, {-1, -1, 21}},
+    {{22, 23, -1}, {24, 25, -1}, {-1, -1, -1}},
+    {{26, -1, 27}, {-1, 30, -1}, {28, -1, 29}},
+    {{35, -1, -1}, {-1, 31, 32}, {-1, 33, 34}},
+    {{36, 37, 38}, {39, 40, 41}, {42, 43, 44}}};
+
+// Value of lane (l ^ M) within each group of 8 lanes, M in 1..7: a DPP quad
+// permutation for M < 4; the half-row mirror (l ^ 7) followed by the quad
+// permutation of M ^ 7 for M >= 4.  Every lane of the wave must be active.
+template <int M>
+__device__ __forceinline__ int lane_xor_i(int v) {
+  static_assert(M > 0 && M < 8, "lanes of one 8-lane group");
+  constexpr int kQuad[4] = {0xE4, 0xB1, 0x4E, 0x1B};   // quad_perm: l, l ^ 1, l ^ 2, l ^ 3
+  if constexpr (M >= 4) {
+    v = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    if constexpr ((M ^ 7) != 0) v = __builtin_amdgcn_mov_dpp(v, kQuad[M ^ 7], 0xF, 0xF, false);
+    return v;
+  } else {
+    return __builtin_amdgcn_mov_dpp(v, kQuad[M], 0xF, 0xF, false);
+  }
+}
+template <int M>
+__device__ __forceinline__ float lane_xor(float v) {
+  return __builtin_bit_cast(float, lane_xor_i<M>(__builtin_bit_cast(int, v)));
+}
+template <int M>
+__device__ __forceinline__ double lane_xor(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)lane_xor_i<M>((int)(unsigned)u);
+  const unsigned hi = (unsigned)lane_xor_i<M>((int)(unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// One butterfly of the corner transform across the lanes of an element, on
+// the corner bit M: (lo, hi) -> (lo + hi, hi - lo); the inverse
+// (sm, df) -> (sm - df, sm + df).  The same butterflies as face_fwd / face_inv
+// and the z butterfly of the element kernel.
+template <int M, typename T>
+__device__ __forceinline__ T wht_fwd(T v, bool hi) {
+  const T p = lane_xor<M>(v);
+  return hi ? v - p : v + p;
+}
+template <int M, typename T>
+__device__ __forceinline__ T wht_inv(T v, bool hi) {
+  const T p = lane_xor<M>(v);
+  return hi ? p + v : v - p;
+}
+
+// Lane j's sector rows for the 8-lane element product: output component c,
+// input component c' of the block that couples mode j of c with mode
+// j ^ e_c ^ e_c' of c' (0 where the block is zero).
+template <typename T>
+__device__ __forceinline__ void sec8_rows(const T* sec, int j, T kc[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int c2 = 0; c2 < 3; ++c2) {
+      const int k = kSecIdx[j ^ (1 << c)][c][c2];
+      kc[c][c2] = k >= 0 ? sec[k] : T(0);
+    }
+}
+
+// The element product s KE u over 8 lanes, lane j holding corner j (j = ox +
+// 2 oy + 4 oz) of the element: u, its (masked) corner values, become o, that
+// corner's outputs.  Three butterflies across the 8 lanes give the
+// Walsh-Hadamard modes (lane j holds mode j of each component), the sector
+// blocks couple mode j of component c with modes j ^ e_c ^ e_c' of the others
+// (three lane exchanges), and the inverse butterflies return to corners.
+// Every lane of the wave must take part.
+template <typename T>
+__device__ __forceinline__ void sec8_apply(const T kc[3][3], T s, int j, const T u_in[3], T o[3]) {
+  const bool h1 = j & 1, h2 = j & 2, h4 = j & 4;
+  T u[3] = {u_in[0], u_in[1], u_in[2]};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c] = wht_fwd<1>(u[c], h1);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c] = wht_fwd<2>(u[c], h2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c] = wht_fwd<4>(u[c], h4);
+  const T g30 = lane_xor<3>(u[0]), g31 = lane_xor<3>(u[1]);
+  const T g50 = lane_xor<5>(u[0]), g52 = lane_xor<5>(u[2]);
+  const T g61 = lane_xor<6>(u[1]), g62 = lane_xor<6>(u[2]);
+  o[0] = s * (kc[0][0] * u[0] + kc[0][1] * g31 + kc[0][2] * g52);
+  o[1] = s * (kc[1][0] * g30 + kc[1][1] * u[1] + kc[1][2] * g62);
+  o[2] = s * (kc[2][0] * g50 + kc[2][1] * g61 + kc[2][2] * u[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = wht_inv<1>(o[c], h1);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = wht_inv<2>(o[c], h2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = wht_inv<4>(o[c], h4);
 }
 
 // Reciprocal of the on-the-fly diagonal: the hardware reciprocal in FP32

@@ -36,59 +36,6 @@ constexpr int kTailThreads = 1024;
 constexpr int kTailElPass = 2;    // element passes: 128 elements of 8 lanes each per pass
 constexpr int kTailDofPass = 2;   // DOFs owned per thread
 
-// Index of sector entry (chi, row component c, column component c') in the
-// 45 entries of tk_sector.hpp, -1 where the block is zero.
-__constant__ signed char kTailSecIdx[8][3][3] = {
-    {{0, 1, 2}, {3, 4, 5}, {6, 7, 8}},
-    {{-1, -1, -1}, {-1, 9, 10}, {-1, 11, 12}},
-    {{13, -1, 14}, {-1, -1, -1}, {15, -1, 16}},
-    {{17, 18, -1}, {19, 20, -1}, {-1, -1, 21}},
-    {{22, 23, -1}, {24, 25, -1}, {-1, -1, -1}},
-    {{26, -1, 27}, {-1, 30, -1}, {28, -1, 29}},
-    {{35, -1, -1}, {-1, 31, 32}, {-1, 33, 34}},
-    {{36, 37, 38}, {39, 40, 41}, {42, 43, 44}}};
-
-// Value of lane (l ^ M) within each group of 8 lanes, M in 1..7: a DPP quad
-// permutation for M < 4; the half-row mirror (l ^ 7) followed by the quad
-// permutation of M ^ 7 for M >= 4.  Every lane of the wave must be active.
-template <int M>
-__device__ __forceinline__ int lane_xor_i(int v) {
-  static_assert(M > 0 && M < 8, "lanes of one 8-lane group");
-  constexpr int kQuad[4] = {0xE4, 0xB1, 0x4E, 0x1B};   // quad_perm: l, l ^ 1, l ^ 2, l ^ 3
-  if constexpr (M >= 4) {
-    v = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
-    if constexpr ((M ^ 7) != 0) v = __builtin_amdgcn_mov_dpp(v, kQuad[M ^ 7], 0xF, 0xF, false);
-    return v;
-  } else {
-    return __builtin_amdgcn_mov_dpp(v, kQuad[M], 0xF, 0xF, false);
-  }
-}
-template <int M>
-__device__ __forceinline__ float lane_xor(float v) {
-  return __builtin_bit_cast(float, lane_xor_i<M>(__builtin_bit_cast(int, v)));
-}
-template <int M>
-__device__ __forceinline__ double lane_xor(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)lane_xor_i<M>((int)(unsigned)u);
-  const unsigned hi = (unsigned)lane_xor_i<M>((int)(unsigned)(u >> 32));
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-// One butterfly of the corner transform across the lanes of an element, on
-// the corner bit M: (lo, hi) -> (lo + hi, hi - lo); the inverse
-// (sm, df) -> (sm - df, sm + df).  The same butterflies as face_fwd / face_inv
-// and the z butterfly of the element kernel.
-template <int M, typename T>
-__device__ __forceinline__ T wht_fwd(T v, bool hi) {
-  const T p = lane_xor<M>(v);
-  return hi ? v - p : v + p;
-}
-template <int M, typename T>
-__device__ __forceinline__ T wht_inv(T v, bool hi) {
-  const T p = lane_xor<M>(v);
-  return hi ? p + v : v - p;
-}
-
 // LDS layout of the tail for level g and coarsest grid gc (element type T),
 // offsets in elements of T: x, the residual, the element outputs (24 rows of
 // ne_pad, then one zero), coarse right-hand side and solution, the coarse
@@ -141,13 +88,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   const int j = tid & 7;
   const int npass = (g.nele + 127) >> 7;
   T kc[3][3];   // this lane's sector rows: output component c, input component c'
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int c2 = 0; c2 < 3; ++c2) {
-      const int k = kTailSecIdx[j ^ (1 << c)][c][c2];
-      kc[c][c2] = k >= 0 ? sec[k] : T(0);
-    }
+  sec8_rows(sec, j, kc);
   int nd[kTailElPass];       // corner node of the lane's element (-1: no element)
   T se[kTailElPass];         // element scale
 #pragma unroll
@@ -208,39 +149,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   for (int p = 0; p < kTailElPass; ++p) cfix[p] = nd[p] >= 0 ? F[nd[p]] : 7;
 
   // EY = the element products s_e KE u_e (inputs masked at fixed DOFs) in the
-  // sector form: each lane loads its corner, three butterflies across the 8
-  // lanes give the Walsh-Hadamard modes (lane j holds mode j of each
-  // component), the sector blocks couple mode j of component c with modes
-  // j ^ e_c ^ e_c' of the others (three lane exchanges), and the inverse
-  // butterflies return corner j's outputs
-  const bool h1 = j & 1, h2 = j & 2, h4 = j & 4;
+  // sector form, 8 lanes per element (sec8_apply)
   auto element_products = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int p = 0; p < kTailElPass; ++p) {
       if (p >= npass) break;   // uniform: every lane of the wave takes part
-      T u[3];
+      T u[3], o[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) u[c] = (nd[p] >= 0 && !((cfix[p] >> c) & 1)) ? X[3 * nd[p] + c] : T(0);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<1>(u[c], h1);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<2>(u[c], h2);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = wht_fwd<4>(u[c], h4);
-      const T g30 = lane_xor<3>(u[0]), g31 = lane_xor<3>(u[1]);
-      const T g50 = lane_xor<5>(u[0]), g52 = lane_xor<5>(u[2]);
-      const T g61 = lane_xor<6>(u[1]), g62 = lane_xor<6>(u[2]);
-      const T s = se[p];
-      T o[3];
-      o[0] = s * (kc[0][0] * u[0] + kc[0][1] * g31 + kc[0][2] * g52);
-      o[1] = s * (kc[1][0] * g30 + kc[1][1] * u[1] + kc[1][2] * g62);
-      o[2] = s * (kc[2][0] * g50 + kc[2][1] * g61 + kc[2][2] * u[2]);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o[c] = wht_inv<1>(o[c], h1);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o[c] = wht_inv<2>(o[c], h2);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o[c] = wht_inv<4>(o[c], h4);
+      sec8_apply(kc, se[p], j, u, o);
       const int e = p * 128 + (tid >> 3);
       if (nd[p] >= 0)
 #pragma unroll
