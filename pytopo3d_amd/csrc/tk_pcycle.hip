@@ -145,9 +145,11 @@ __device__ __forceinline__ void pc_reset_other(unsigned* flags, unsigned* ticket
 // Rank of this workgroup among the participants (tickets from *ticket), or
 // -1: not on XCD 0, stopped solver, or a 33rd block on XCD 0.
 __device__ int pc_rank(unsigned* ticket, const int* done, Scalars* sc) {
-  if (pc_xcc() != 0 || *done || sc->pc_err) return -1;
+  if (pc_xcc() != 0) return -1;   // the same for every thread of the workgroup
+  // one thread reads the flags (another workgroup of the launch may be
+  // setting them) and decides for the whole workgroup
   __shared__ int rank;
-  if (threadIdx.x == 0) rank = (int)atomicAdd(ticket, 1u);
+  if (threadIdx.x == 0) rank = (*done || sc->pc_err) ? kPcWG : (int)atomicAdd(ticket, 1u);
   __syncthreads();
   return rank < kPcWG ? rank : -1;
 }
