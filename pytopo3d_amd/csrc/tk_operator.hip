@@ -360,16 +360,59 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid f, Grid c,
   restrict_node(f, c, fc, rf, bc, I);
 }
 
+// x_f += P x_c at the fine nodes (2Y, ix, iz) and (2Y + 1, ix, iz): both
+// interpolate between coarse y rows Y and Y + 1 (the even node with weights 1
+// and 0), so the pair shares one set of 8 coarse gathers, interpolated in x
+// and z first.  Free DOFs only.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_prolong_add(Grid c, Grid f,
-                                                         const uint8_t* __restrict__ ff,
-                                                         const T* __restrict__ xc,
-                                                         T* __restrict__ xf,
-                                                         const int* __restrict__ done) {
+__global__ __launch_bounds__(kBlock) void k_prolong_add2(Grid c, Grid f,
+                                                          const uint8_t* __restrict__ ff,
+                                                          const T* __restrict__ xc,
+                                                          T* __restrict__ xf,
+                                                          const int* __restrict__ done) {
   if (done && *done) return;
+  const int npy = (f.nny + 1) >> 1;
   const int i = xcd_logical(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= f.nnode) return;
-  prolong_node(c, f, ff, xc, xf, i);
+  if (i >= npy * f.nnx * f.nnz) return;
+  const int Y = i % npy, t = i / npy, ix = t % f.nnx, iz = t / f.nnx;
+  const int Y1 = Y + 1 < c.nny ? Y + 1 : Y;
+  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, z0 = iz >> 1, z1 = (iz + 1) >> 1;
+  const T wx[2] = {(ix & 1) ? T(0.5) : T(1), (ix & 1) ? T(0.5) : T(0)};
+  const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
+  const int cxy = c.nnx * c.nny;
+  T a0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int kx = k & 1, kz = k >> 1;
+    const int base = (kx ? x1 : x0) * c.nny + (kz ? z1 : z0) * cxy;
+    const T w = wx[kx] * wz[kz];
+    const T* q0 = xc + 3 * (size_t)(Y + base);
+    const T* q1 = xc + 3 * (size_t)(Y1 + base);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      a0[r] += w * q0[r];
+      a1[r] += w * q1[r];
+    }
+  }
+  const int n0 = 2 * Y + ix * f.nny + iz * f.nnx * f.nny;
+  // both nodes' values loaded before either is stored (unconditional loads;
+  // the odd node, when the grid has none, at the even one's clamped index)
+  const bool odd = 2 * Y + 1 < f.nny;
+  const int n1 = odd ? n0 + 1 : n0;
+  T* o0 = xf + 3 * (size_t)n0;
+  T* o1 = xf + 3 * (size_t)n1;
+  const uint8_t f0 = ff[n0], f1 = ff[n1];
+  T v0[3], v1[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    v0[r] = o0[r];
+    v1[r] = o1[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o0[r] = ((f0 >> r) & 1) ? v0[r] : v0[r] + a0[r];
+  if (odd)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) o1[r] = ((f1 >> r) & 1) ? v1[r] : v1[r] + T(0.5) * (a0[r] + a1[r]);
 }
 
 // ------------------------------------------------------------------ coarsest grid
@@ -627,7 +670,8 @@ void launch_restrict(const Level& f, const Level& c, const T* rf, T* bc, hipStre
 template <typename T>
 void launch_prolong_add(const Level& c, const Level& f, const T* xc, T* xf, hipStream_t s,
                         const int* done) {
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblocks(f.g.nnode)), dim3(kBlock), 0, s, c.g, f.g,
+  const int pairs = (f.g.nny + 1) / 2 * f.g.nnx * f.g.nnz;
+  hipLaunchKernelGGL(k_prolong_add2<T>, dim3(nblocks(pairs)), dim3(kBlock), 0, s, c.g, f.g,
                      f.fixed, xc, xf, done);
 }
 

@@ -72,38 +72,4 @@ __device__ __forceinline__ void restrict_node(const Grid& f, const Grid& c,
   out[2] = o[2];
 }
 
-// x_f += P x_c on free fine DOFs.  Branch-free: an even fine index has one
-// coarse parent per axis, given weight 1 and a duplicate with weight 0.
-template <typename T>
-__device__ __forceinline__ void prolong_node(const Grid& c, const Grid& f,
-                                             const uint8_t* __restrict__ ff, const T* xc, T* xf,
-                                             int i) {
-  int iy, ix, iz;
-  grid_node_coords(f, i, iy, ix, iz);
-  const int y0 = iy >> 1, y1 = (iy + 1) >> 1;
-  const int x0 = ix >> 1, x1 = (ix + 1) >> 1;
-  const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
-  const T wy[2] = {(iy & 1) ? T(0.5) : T(1), (iy & 1) ? T(0.5) : T(0)};
-  const T wx[2] = {(ix & 1) ? T(0.5) : T(1), (ix & 1) ? T(0.5) : T(0)};
-  const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
-  const int cxy = c.nnx * c.nny;
-  T a0 = T(0), a1 = T(0), a2 = T(0);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
-    const int I = (ky ? y1 : y0) + (kx ? x1 : x0) * c.nny + (kz ? z1 : z0) * cxy;
-    const T w = wy[ky] * wx[kx] * wz[kz];
-    const T* q = xc + 3 * (size_t)I;
-    a0 += w * q[0];
-    a1 += w * q[1];
-    a2 += w * q[2];
-  }
-  const uint8_t fi = ff[i];
-  T* o = xf + 3 * (size_t)i;
-  const T v0 = o[0], v1 = o[1], v2 = o[2];
-  o[0] = (fi & 1) ? v0 : v0 + a0;
-  o[1] = (fi & 2) ? v1 : v1 + a1;
-  o[2] = (fi & 4) ? v2 : v2 + a2;
-}
-
 }  // namespace tk
