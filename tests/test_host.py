@@ -102,9 +102,19 @@ _C_CALLER = r"""
 int main(void) {
   tk_options o;
   tk_options_default(&o);
-  /* a call that fails in argument checking, before any device work */
+  /* calls that fail in argument checking, before any device work */
   int rc = tk_plan_create(NULL, 4, 4, 4, NULL, 1.5, &o);
   printf("%d %d %d %s\n", o.smooth_degree, o.coarse_degree, rc, tk_last_error());
+  static double KE[576];
+  for (int i = 0; i < 24; ++i) KE[i * 25] = 1.0;
+  tk_plan* p = NULL;
+  o.mid_cheb_lo = 2.0;   /* above cheb_hi */
+  rc = tk_plan_create(&p, 4, 4, 4, KE, 1.5, &o);
+  printf("| %d %d %s\n", rc, p == NULL, tk_last_error());
+  tk_options_default(&o);
+  o.direction_slots = 1;
+  rc = tk_plan_create(&p, 4, 4, 4, KE, 1.5, &o);
+  printf("| %d %d %s\n", rc, p == NULL, tk_last_error());
   return 0;
 }
 """
@@ -124,9 +134,16 @@ def test_header_is_plain_c_and_links_from_c(tmp_path):
                     "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
                     "-L", libdir, "-lpytopo3d_hip", "-Wl,-rpath," + libdir],
                    check=True, capture_output=True, text=True)
-    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    lines = subprocess.run([str(exe)], check=True, capture_output=True,
+                           text=True).stdout.split("| ")
+    out = lines[0].split()
     assert out[:3] == ["2", "12", "1"]
     assert "out must not be NULL" in " ".join(out[3:])
+    # option checks (smoothing intervals, direction slots) refuse before any allocation
+    assert lines[1].split()[:2] == ["1", "1"]
+    assert "mid_cheb_lo must be below cheb_hi" in lines[1]
+    assert lines[2].split()[:2] == ["1", "1"]
+    assert "direction_slots must be 0 or in [2, " in lines[2]
 
 
 def test_product_refuses_to_run_without_a_gpu(monkeypatch):
