@@ -60,8 +60,6 @@ static void free_plan(Plan* P) {
       for (int c = 0; c < Plan::kChunkSizes; ++c)
         for (int d = 0; d < 2; ++d)
           if (P->gexec[a][b][c][d]) (void)hipGraphExecDestroy(P->gexec[a][b][c][d]);
-  if (P->gev_in) (void)hipEventDestroy(P->gev_in);
-  if (P->gev_out) (void)hipEventDestroy(P->gev_out);
   if (P->gstream) (void)hipStreamDestroy(P->gstream);
   if (P->cstream) (void)hipStreamDestroy(P->cstream);
   if (P->cev_in) (void)hipEventDestroy(P->cev_in);
@@ -242,7 +240,8 @@ static int create(Plan*& out, int nelx, int nely, int nelz, const double* KE, do
     free_plan(P);
     return 2;
   }
-  if (hipHostMalloc((void**)&P->sc_host, sizeof(Scalars)) != hipSuccess) {
+  if (hipHostMalloc((void**)&P->sc_host, sizeof(Scalars)) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&P->sc_host_dev, P->sc_host, 0) != hipSuccess) {
     free_plan(P);
     set_error("hipHostMalloc failed");
     return 2;

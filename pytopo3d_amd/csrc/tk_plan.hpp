@@ -123,6 +123,7 @@ struct Plan {
   int partials_len = 0;
   Scalars* sc = nullptr;          // device scalars
   Scalars* sc_host = nullptr;     // pinned mirror
+  Scalars* sc_host_dev = nullptr; // its device address (the captured chunks write it)
   Options opt;
   Counters cnt;
   // bookkeeping of the last set-up (TK_SOLVE_REUSE must match all of it)
@@ -133,9 +134,8 @@ struct Plan {
   long long setup_mask_gen = -1;  // mask_gen at the last set-up
   // captured Krylov chunks, [precision][jacobi]: every launch argument of a
   // chunk is plan-owned (the iterate lives in x64), so a chunk is captured
-  // once per plan and replayed on gstream.
+  // once per plan (on gstream) and replayed on the caller's stream.
   hipStream_t gstream = nullptr;
-  hipEvent_t gev_in = nullptr, gev_out = nullptr;
   // the coarsest factorisation runs on cstream beside the rest of the set-up;
   // the first coarsest solve after it waits for cev_out (coarse_pending)
   hipStream_t cstream = nullptr;

@@ -265,6 +265,17 @@ template <typename T> static LevelBuf<T>& buf(Level& L);
 template <> LevelBuf<double>& buf<double>(Level& L) { return L.f64; }
 template <> LevelBuf<float>& buf<float>(Level& L) { return L.f32; }
 
+// The solver scalars copied to the host mirror by a kernel (one captured as
+// the last node of every Krylov chunk, so the host only waits for the chunk).
+__global__ __launch_bounds__(64) void k_scalars_out(const Scalars* __restrict__ sc,
+                                                     Scalars* __restrict__ host) {
+  constexpr int n = (int)(sizeof(Scalars) / sizeof(int));
+  static_assert(sizeof(Scalars) % sizeof(int) == 0, "whole words");
+  const int* a = reinterpret_cast<const int*>(sc);
+  int* b = reinterpret_cast<int*>(host);
+  for (int i = threadIdx.x; i < n; i += 64) b[i] = a[i];
+}
+
 static int read_scalars(Plan& P, hipStream_t s) {
   TK_HIP(hipMemcpyAsync(P.sc_host, P.sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   TK_HIP(hipStreamSynchronize(s));
@@ -487,10 +498,8 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
                       hipStream_t s) {
   bool graphs = !P.opt.no_graphs;
   if (graphs && !P.gstream) {
-    if (hipStreamCreateWithFlags(&P.gstream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&P.gev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&P.gev_out, hipEventDisableTiming) != hipSuccess) {
-      set_error("could not create the solver stream");
+    if (hipStreamCreateWithFlags(&P.gstream, hipStreamNonBlocking) != hipSuccess) {
+      set_error("could not create the capture stream");
       return 2;
     }
   }
@@ -507,6 +516,7 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
       const Counters before = P.cnt;
       TK_HIP(hipStreamBeginCapture(P.gstream, hipStreamCaptureModeThreadLocal));
       body(P.gstream, m, par);
+      hipLaunchKernelGGL(k_scalars_out, dim3(1), dim3(64), 0, P.gstream, P.sc, P.sc_host_dev);
       hipGraph_t g = nullptr;
       TK_HIP(hipStreamEndCapture(P.gstream, &g));
       const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -532,13 +542,11 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
       body(s, sizes[k], par);
       return 0;
     }
+    // the chunk runs on the caller's stream; its last kernel has copied the
+    // scalars to the host mirror
     hipGraphExec_t& ex = P.gexec[pa][ja][k][par];
     Counters& delta = P.gdelta[pa][ja][k][par];
-    TK_HIP(hipEventRecord(P.gev_in, s));
-    TK_HIP(hipStreamWaitEvent(P.gstream, P.gev_in, 0));
-    TK_HIP(hipGraphLaunch(ex, P.gstream));
-    TK_HIP(hipEventRecord(P.gev_out, P.gstream));
-    TK_HIP(hipStreamWaitEvent(s, P.gev_out, 0));
+    TK_HIP(hipGraphLaunch(ex, s));
     add_counters(P.cnt, delta, 1);
     return 0;
   };
@@ -551,7 +559,11 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
   for (long long guard = 0; guard < (long long)maxit + 8; ++guard) {
     if (int rc = run(k, par)) return rc;
     if (parity) par ^= sizes[k] & 1;
-    if (read_scalars(P, s)) return 2;
+    if (graphs) {
+      TK_HIP(hipStreamSynchronize(s));
+    } else if (read_scalars(P, s)) {
+      return 2;
+    }
     const Scalars& H = *P.sc_host;
     if (H.done) break;
     if (rr_prev < 0.0) rr_prev = H.rr0;
