@@ -610,24 +610,25 @@ static int pcg(Plan& P, const T* b, T* x, T* r, T* p, T* q, int x_zero, double r
     }
     finalize(P, kFinRz, G, st, 0.0, first);
   };
-  // z buffer: q is free until the first operator application
-  int G = 0;
-  const T* z = precondition<T>(P, r, q, jacobi, s, &G);
-  reduce_rz(z, G, 1, s);
-  hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, p, 1, done, P.sc);
+  // One iteration: z = M r (into q when the preconditioner needs a buffer: q
+  // is read by nothing after the update), r.z and beta, p = z + beta p,
+  // q = K p, alpha, x and r, and the convergence test last -- so a chunk
+  // ends right after a test, and the solve that converges at a chunk's end
+  // runs no preconditioner for nothing.
+  auto iteration = [&](int first, hipStream_t st) {
+    int Gk = 0;
+    const T* zz = precondition<T>(P, r, q, jacobi, st, &Gk);
+    reduce_rz(zz, Gk, first, st);
+    hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, zz, p, first, done,
+                       P.sc);
+    launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, st);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, p, q,
+                       done, P.partials, P.sc);
+    finalize(P, kFinUpdate, red_blocks(n), st);
+  };
+  iteration(1, s);
   auto body = [&](hipStream_t st, int m, int) {
-    for (int k = 0; k < m; ++k) {
-      launch_apply<T>(P, L0, B0.s, p, q, nullptr, 1, st);
-      hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(n)), dim3(kBlock), 0, st, n, x, r, p, q,
-                         done, P.partials, P.sc);
-      finalize(P, kFinUpdate, red_blocks(n), st);
-      // the preconditioner output must not alias q (read by nothing after the update)
-      int Gk = 0;
-      const T* zz = precondition<T>(P, r, q, jacobi, st, &Gk);
-      reduce_rz(zz, Gk, 0, st);
-      hipLaunchKernelGGL(k_cg_dir<T>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, zz, p, 0, done,
-                         P.sc);
-    }
+    for (int k = 0; k < m; ++k) iteration(0, st);
   };
   int rc = run_chunks(P, 0, jacobi, maxit, -1, body, s);
   if (rc) return rc;
@@ -703,12 +704,14 @@ static int pcg_mixed(Plan& P, const double* b, double* x, int x_zero, double rto
                        q, r32, z32, done, upd_partials, P.sc, ring ? P.pring : nullptr,
                        P.ring_m, P.alpha_hist, P.partials, G_pq);
     finalize(P, kFinUpdateFlex, red_blocks(n), st, 0.0, 0, upd_partials);
-    z32 = correction(0, st);
   };
   iteration(pbuf[0], nullptr, s);
+  // the correction opens each chunk iteration, so a chunk ends right after a
+  // convergence test (no preconditioner runs for nothing after the last one)
   auto body = [&](hipStream_t st, int m, int par) {
     for (int k = 0; k < m; ++k) {
       const int i = par + k;   // parity of the global iteration index
+      z32 = correction(0, st);
       iteration(pbuf[i & 1], pbuf[(i + 1) & 1], st);
     }
   };
