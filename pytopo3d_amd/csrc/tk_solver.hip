@@ -550,15 +550,23 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
     add_counters(P.cnt, delta, 1);
     return 0;
   };
-  int k = 1;                       // first chunk: base iterations
+  int need = sizes[1];             // first round: base iterations
   int par = parity ? par0 : 0;
   // the decay is measured from the initial residual for the first prediction
   // (kFinNorms left it in rr0; the iteration count starts at 0)
   int it_prev = 0;
   double rr_prev = -1.0;
   for (long long guard = 0; guard < (long long)maxit + 8; ++guard) {
-    if (int rc = run(k, par)) return rc;
-    if (parity) par ^= sizes[k] & 1;
+    // one round of `need` iterations: the captured chunks whose lengths sum
+    // to it, launched back to back, then one look at the scalars
+    for (int left = need; left > 0;) {
+      int k = Plan::kChunkSizes - 1;
+      for (int j = 0; j < Plan::kChunkSizes; ++j)
+        if (sizes[j] <= left) { k = j; break; }
+      if (int rc = run(k, par)) return rc;
+      if (parity) par ^= sizes[k] & 1;
+      left -= sizes[k];
+    }
     if (graphs) {
       TK_HIP(hipStreamSynchronize(s));
     } else if (read_scalars(P, s)) {
@@ -569,7 +577,7 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
     if (rr_prev < 0.0) rr_prev = H.rr0;
     // iterations still needed at the observed residual decay rate
     const double target = H.tol2 * H.bb;
-    int need = sizes[0];
+    need = sizes[0];
     if (it_prev < H.iter && rr_prev > 0.0 && H.rr > 0.0 && H.rr < rr_prev && target > 0.0) {
       const double per_it = std::log(H.rr / rr_prev) / (double)(H.iter - it_prev);
       const double est = 0.75 * std::log(target / H.rr) / per_it;
@@ -577,9 +585,6 @@ static int run_chunks(Plan& P, int prec, int jacobi, int maxit, int par0, Body b
     }
     it_prev = H.iter;
     rr_prev = H.rr;
-    k = Plan::kChunkSizes - 1;
-    for (int j = 0; j < Plan::kChunkSizes; ++j)
-      if (sizes[j] <= need) { k = j; break; }
   }
   return 0;
 }
