@@ -139,9 +139,11 @@ class SIMPProblem:
         self.last_iters, self.last_relres = it, rr
         return U
 
-    def sensitivities(self, U, rho, drho, penal, dc_dx=None, dv_dx=None, ce=None, dc_drho=None):
+    def sensitivities(self, U, rho, drho, penal, dc_dx=None, dv_dx=None, ce=None, dc_drho=None,
+                      sync=True):
+        """c (sync) or None: then c stays on the device for Plan.results()."""
         c = self.plan.energy(U, rho, penal, drho=drho, ce=ce, dc_drho=dc_drho, dc_dx=dc_dx,
-                             dv_dx=dv_dx, sync=True)
+                             dv_dx=dv_dx, sync=sync)
         return c
 
     def close(self):
@@ -214,7 +216,8 @@ def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float
     t0 = time.perf_counter()
     U = problem.solve(st.rho, penal, **solve_kw)
     t1 = time.perf_counter()
-    c = problem.sensitivities(U, st.rho, st.drho, penal, dc_dx=st.dc_dx, dv_dx=st.dv_dx)
+    # c is read with the step's other results (one wait for the device)
+    problem.sensitivities(U, st.rho, st.drho, penal, dc_dx=st.dc_dx, dv_dx=st.dv_dx, sync=False)
     t2 = time.perf_counter()
     plan.oc_update(st.x, st.dc_dx, st.dv_dx, move, target, beta, eta, st.x_new)
     if st.snapshots is not None:
@@ -225,7 +228,8 @@ def design_step(problem: SIMPProblem, st: DesignState, penal: float, beta: float
     t3 = time.perf_counter()
     st.x, st.x_new = st.x_new, st.x
     st.rho, st.rho_alt = st.rho_alt, st.rho
-    return {"c": c, "change": float(res["change"]), "vol": res["vol_sum"] / problem.nfree,
+    return {"c": float(res["c"]), "change": float(res["change"]),
+            "vol": res["vol_sum"] / problem.nfree,
             "gray": res["gray_cnt"] / problem.nfree, "oc_steps": int(res["oc_steps"]),
             "oc_evals": int(res["oc_evals"]),
             "solve_iters": problem.last_iters, "relres": problem.last_relres,
