@@ -207,6 +207,15 @@ int tk_apply_K(tk_plan* plan, int prec, const double* rho, double penal, const v
  * (dimensions from tk_plan_levels) of that precision.  No counterpart in the
  * reference (it has no multigrid); it lets tests check each level's operator. */
 int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, void* stream);
+/* The V-cycle's grid transfers between level `level` and the next coarser one
+ * (precision prec, values of that precision, node counts from tk_plan_levels;
+ * the plan's masks must be set): direction 0, out (coarse) = R in, the
+ * trilinear restriction R = P^T of a fine field that is zero on the fine fixed
+ * DOFs, zero on the coarse fixed DOFs; direction 1, out (fine) += P in, the
+ * trilinear prolongation, added on the fine free DOFs only.  No counterpart in
+ * the reference (it has no multigrid); it lets tests check the transfers. */
+int tk_transfer_apply(tk_plan* plan, int prec, int level, int direction, const void* in,
+                      void* out, void* stream);
 /* x = the part of the V-cycle that one workgroup runs on the tail level (the
  * second-coarsest level): pre-smoothing from x = 0, residual, restriction,
  * coarsest solve with the stored inverse, prolongation and post-smoothing,

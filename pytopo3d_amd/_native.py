@@ -74,6 +74,7 @@ _SIGS = {
     "tk_plan_tail_level": (_I, [_P, ctypes.POINTER(_I)]),
     "tk_plan_persistent_levels": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "tk_coarse_cycle_apply": (_I, [_P, _I, _P, _P, _P]),
+    "tk_transfer_apply": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "tk_plan_set_barrier_timeout": (_I, [_P, _D]),
     "tk_filter_row_sums": (_I, [_P, _P, _P]),
     "tk_filter_project": (_I, [_P, _P, _D, _D, _P, _P, _P, _P]),
@@ -245,6 +246,12 @@ class Plan:
         f, n = _I(), _I()
         _check(lib().tk_plan_persistent_levels(self._h, ctypes.byref(f), ctypes.byref(n)))
         return int(f.value), int(n.value)
+
+    def transfer_apply(self, level, direction, src, dst, precision="fp64"):
+        """dst = R src (direction 0, level -> level + 1) or dst += P src
+        (direction 1, level + 1 -> level)."""
+        _check(lib().tk_transfer_apply(self._h, PRECISIONS[precision], int(level), int(direction),
+                                       _ptr(src), _ptr(dst), _stream()))
 
     def set_barrier_timeout(self, seconds):
         """Wait limit of the persistent coarse cycle's barriers (default 0.05 s)."""

@@ -587,6 +587,43 @@ int tk_level_apply(tk_plan* plan, int prec, int level, const void* u, void* y, v
   return tk::apply_level(P, prec, level, u, y, (hipStream_t)stream);
 }
 
+int tk_transfer_apply(tk_plan* plan, int prec, int level, int direction, const void* in,
+                      void* out, void* stream) {
+  TK_PLAN_OR_FAIL(plan);
+  if (!in || !out) {
+    set_error("in and out are required");
+    return 1;
+  }
+  if (prec != TK_FP64 && prec != TK_MIXED_FP32) {
+    set_error("unknown precision");
+    return 1;
+  }
+  if (level < 0 || level + 1 >= P.nlev) {
+    set_error("level must be in [0, " + std::to_string(P.nlev - 1) + ")");
+    return 1;
+  }
+  if (direction != 0 && direction != 1) {
+    set_error("direction must be 0 (restriction) or 1 (prolongation)");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const tk::Level& F = P.lv[level];
+  const tk::Level& C = P.lv[level + 1];
+  if (prec == TK_FP64) {
+    if (direction == 0)
+      tk::launch_restrict<double>(F, C, (const double*)in, (double*)out, s, nullptr);
+    else
+      tk::launch_prolong_add<double>(C, F, (const double*)in, (double*)out, s, nullptr);
+  } else {
+    if (direction == 0)
+      tk::launch_restrict<float>(F, C, (const float*)in, (float*)out, s, nullptr);
+    else
+      tk::launch_prolong_add<float>(C, F, (const float*)in, (float*)out, s, nullptr);
+  }
+  TK_LAUNCH_CHECK();
+  return 0;
+}
+
 int tk_plan_tail_level(const tk_plan* plan, int* level) {
   if (!plan || !plan->p || !level) {
     set_error("null plan or level");
