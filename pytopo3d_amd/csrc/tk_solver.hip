@@ -221,6 +221,9 @@ __global__ __launch_bounds__(kBlock) void k_ring_fold_out(int n, const double* _
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double xi = x[i];
+  // unrolled so that the loads of several slots are in flight together (the
+  // sum keeps the slot order)
+#pragma unroll 8
   for (int j = 0; j < c; ++j) xi += alpha_hist[j] * (double)ring[(long long)j * n + i];
   U[i] = xi;
 }
