@@ -357,7 +357,10 @@ def _cpu_baseline(args):
                    f"{n1} at {'x'.join(map(str, d1))} in {t1:.1f} s ({1 / s1:.3f} iters/s), "
                    f"{n2} at {'x'.join(map(str, d2))} in {t2:.1f} s ({1 / s2:.3f} iters/s); "
                    f"time per iteration grows as elements^{alpha:.2f} between them, which "
-                   f"carries the rate to {args.nelx}x{args.nely}x{args.nelz}"),
+                   f"carries the rate to {args.nelx}x{args.nely}x{args.nelz}; an upper bound "
+                   f"on the CPU rate: one iteration at 60x30x20 took 447 s on one core "
+                   f"(measured once, outside this run), 10x what this exponent predicts, "
+                   f"as the direct solve's fill grows faster on larger grids"),
         "measured": [{"grid": list(d), "iters": n, "seconds": t} for d, n, t in pts],
         "exponent": alpha,
     }
